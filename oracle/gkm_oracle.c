@@ -1,0 +1,793 @@
+/*
+ * gkm_oracle.c -- FP64 CPU restatement of the reference's GKM-WENO step and
+ * Lagrangian particle tracker (nishantp9/DNS_GKM_LPT, src/).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP product path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it.  Nothing under dns_gkm_lpt_amd/ links, loads or calls it.
+ *
+ * Each function cites the reference file:line it restates.  The arithmetic is
+ * written in the reference's evaluation order where that order is the natural
+ * reading of the formula, so that on x86-64 (no FMA contraction, -ffp-contract=off)
+ * the restatement tracks the compiled reference to the last bit or so.
+ *
+ * Pinning (what checks each part, see tests/):
+ *   c2p, flux, WENO, the three sweeps and update: the reference's own
+ *     GKMflux.cpp / update.cpp compiled by oracle/Makefile into oracle/_ref
+ *     (vectors committed under tests/golden/), plus closed forms (uniform state
+ *     -> dt * Euler flux; polynomial exactness of WENO; conservation).
+ *   halo fill, CFL: closed forms.
+ *   natural tricubic B-spline: uniqueness of the natural interpolant, checked
+ *     against scipy.interpolate.CubicSpline(bc_type='natural') axis by axis.
+ *   einspline's exact rounding is NOT pinned (einspline is not in the image).
+ *   particle Heun step: uniform-field translation and linear-field closed form.
+ *   field/particle statistics: analytic sinusoids.
+ */
+#include "gkm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI (atan(1.0) * 4.0) /* param.h:17 */
+
+static inline long pidx(int nt, int j, int i, int k) { return ((long)j * nt + i) * nt + k; }
+static inline double sq(double a) { return a * a; }
+
+/* ------------------------------------------------------------------------- */
+/* GKMflux.cpp:302-309 -- conservative -> primitive                           */
+/* ------------------------------------------------------------------------- */
+void orc_c2p(const double W[5], double gam, double *den, double U[3], double *P)
+{
+    double d = W[0];
+    U[0] = W[1] / d;
+    U[1] = W[2] / d;
+    U[2] = W[3] / d;
+    *P = (d * (gam - 1)) * (W[4] / d - 0.5 * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]));
+    *den = d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* update.cpp:556-680 -- WENO-SYMOO, one side of one interface.               */
+/* a..f are cells i-2..i+3; returns the reconstruction at i+1/2 seen from the  */
+/* left.  The right-side state is the same formula on the mirrored stencil.   */
+/* ------------------------------------------------------------------------- */
+static double weno_side(double a, double b, double c, double d, double e, double f, double eps)
+{
+    const double g0 = 1.0 / 20.0, g1 = 9.0 / 20.0, g2 = 9.0 / 20.0, g3 = 1.0 / 20.0;
+    double q0 = a / 3 - 7 * b / 6 + 11 * c / 6;
+    double q1 = -b / 6 + 5 * c / 6 + d / 3;
+    double q2 = c / 3 + 5 * d / 6 - e / 6;
+    double q3 = 11 * d / 6 - 7 * e / 6 + f / 3;
+    double s0 = 13 * sq(a - 2 * b + c) / 12 + sq(a - 4 * b + 3 * c) / 4;
+    double s1 = 13 * sq(b - 2 * c + d) / 12 + sq(-b + d) / 4;
+    double s2 = 13 * sq(c - 2 * d + e) / 12 + sq(-3 * c + 4 * d - e) / 4;
+    double s3 = 13 * sq(d - 2 * e + f) / 12 + sq(-5 * d + 8 * e - 3 * f) / 4;
+    /* update.cpp:577-584: the fourth indicator is replaced by the largest one */
+    double smax = s0 > s1 ? s0 : s1;
+    smax = smax > s2 ? smax : s2;
+    smax = smax > s3 ? smax : s3;
+    s3 = smax;
+    double a0 = g0 * (1 / (eps + s0));
+    double a1 = g1 * (1 / (eps + s1));
+    double a2 = g2 * (1 / (eps + s2));
+    double a3 = g3 * (1 / (eps + s3));
+    double sum = a0 + a1 + a2 + a3;
+    return (a0 / sum) * q0 + (a1 / sum) * q1 + (a2 / sum) * q2 + (a3 / sum) * q3;
+}
+
+/* temperature of a conservative state, update.cpp:624-629 */
+static double weno_temp(const double *W, double gam, double R)
+{
+    return (gam - 1) * (W[4] / W[0] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) / (W[0] * W[0])) / R;
+}
+
+void orc_weno(const double *S, const orc_gas *g, double Wl[5], double Wr[5])
+{
+    /* update.cpp:561-621: rho, rho u, rho v, rho w reconstructed directly */
+    for (int q = 0; q < 4; q++) {
+        double a = S[0 * 5 + q], b = S[1 * 5 + q], c = S[2 * 5 + q];
+        double d = S[3 * 5 + q], e = S[4 * 5 + q], f = S[5 * 5 + q];
+        Wl[q] = weno_side(a, b, c, d, e, f, g->eps);
+        Wr[q] = weno_side(f, e, d, c, b, a, g->eps);
+    }
+    /* update.cpp:624-675: energy through temperature */
+    double T[6];
+    for (int s = 0; s < 6; s++) T[s] = weno_temp(S + 5 * s, g->gam, g->R);
+    double Tl = weno_side(T[0], T[1], T[2], T[3], T[4], T[5], g->eps);
+    double Tr = weno_side(T[5], T[4], T[3], T[2], T[1], T[0], g->eps);
+    Wl[4] = Wl[0] * g->R * Tl / (g->gam - 1.0) + 0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
+    Wr[4] = Wr[0] * g->R * Tr / (g->gam - 1.0) + 0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+}
+
+void orc_weno_batch(long n, const double *in, const orc_gas *g, double *out)
+{
+    for (long i = 0; i < n; i++) orc_weno(in + 30 * i, g, out + 10 * i, out + 10 * i + 5);
+}
+
+/* ------------------------------------------------------------------------- */
+/* GKMflux.cpp -- gas-kinetic (BGK) interface flux                            */
+/* ------------------------------------------------------------------------- */
+
+/* Moment tables of one Maxwellian (GKMflux.cpp:45-68, 85-103).
+ * fu/fv/fw: full-space moments <u^n>, <v^n>, <w^n>;  hp/hn: half-space moments
+ * of u over u>0 / u<0;  e2/e4: <xi^2>, <xi^4> of the K internal dof.          */
+typedef struct {
+    double den, U[3], P, lam, e2, e4;
+    double fu[7], fv[7], fw[7], hp[7], hn[7];
+} gstate;
+
+static void gstate_init(gstate *s, const double W[5], const orc_gas *g, int want_p, int want_n)
+{
+    orc_c2p(W, g->gam, &s->den, s->U, &s->P);
+    s->lam = 0.5 * s->den / s->P;
+    const double lam = s->lam;
+    const int K = g->K;
+    s->e2 = K / (2 * lam);
+    s->e4 = 3 * K / (4 * lam * lam) + K * (K - 1) / (4 * lam * lam);
+    double *full[3] = {s->fu, s->fv, s->fw};
+    for (int j = 0; j < 3; j++) {
+        double *f = full[j];
+        f[0] = 1.0;
+        f[1] = s->U[j];
+        for (int i = 2; i < 7; i++) f[i] = s->U[j] * f[i - 1] + f[i - 2] * (i - 1) / (2 * lam);
+    }
+    const double u = s->U[0];
+    const double gauss = 0.5 * exp(-lam * u * u) / sqrt(lam * ORC_PI);
+    if (want_p) {
+        s->hp[0] = 0.5 * (erfc(-sqrt(lam) * u));
+        s->hp[1] = u * s->hp[0] + gauss;
+        for (int i = 2; i < 7; i++) s->hp[i] = u * s->hp[i - 1] + s->hp[i - 2] * (i - 1) / (2 * lam);
+    }
+    if (want_n) {
+        s->hn[0] = 0.5 * (erfc(sqrt(lam) * u));
+        s->hn[1] = u * s->hn[0] - gauss;
+        for (int i = 2; i < 7; i++) s->hn[i] = u * s->hn[i - 1] + s->hn[i - 2] * (i - 1) / (2 * lam);
+    }
+}
+
+/* <u^k v^l w^m psi>, psi = (1,u,v,w,(u^2+v^2+w^2+xi^2)/2); GKMflux.cpp:269-278.
+ * tu is the u-table (full or half), v/w always full.                           */
+static void mom(const double *tu, const gstate *s, int k, int l, int m, double M[5])
+{
+    const double *tv = s->fv, *tw = s->fw;
+    M[0] = tu[k] * tv[l] * tw[m];
+    M[1] = tu[1 + k] * tv[l] * tw[m];
+    M[2] = tu[k] * tv[1 + l] * tw[m];
+    M[3] = tu[k] * tv[l] * tw[1 + m];
+    M[4] = 0.5 * (tu[2 + k] * tv[l] * tw[m] + tu[k] * tv[2 + l] * tw[m] + tu[k] * tv[l] * tw[2 + m] +
+                  tu[k] * tv[l] * tw[m] * s->e2);
+}
+
+/* <u^k v^l w^m psi (a . psi)>; GKMflux.cpp:238-267 */
+static void mom_a(const double *tu, const gstate *s, int k, int l, int m, const double a[5], double M[5])
+{
+    const double *tv = s->fv, *tw = s->fw;
+    const double e2 = s->e2, e4 = s->e4;
+#define T3(p, q, r) (tu[p] * tv[q] * tw[r])
+    double h0 = 0.5 * (T3(2 + k, l, m) + T3(k, 2 + l, m) + T3(k, l, 2 + m) + T3(k, l, m) * e2);
+    M[0] = a[0] * T3(k, l, m) + a[1] * T3(1 + k, l, m) + a[2] * T3(k, 1 + l, m) + a[3] * T3(k, l, 1 + m) + a[4] * h0;
+    double h1 = 0.5 * (T3(3 + k, l, m) + T3(1 + k, 2 + l, m) + T3(1 + k, l, 2 + m) + T3(1 + k, l, m) * e2);
+    M[1] = a[0] * T3(1 + k, l, m) + a[1] * T3(2 + k, l, m) + a[2] * T3(1 + k, 1 + l, m) + a[3] * T3(1 + k, l, 1 + m) + a[4] * h1;
+    double h2 = 0.5 * (T3(2 + k, 1 + l, m) + T3(k, 3 + l, m) + T3(k, 1 + l, 2 + m) + T3(k, 1 + l, m) * e2);
+    M[2] = a[0] * T3(k, 1 + l, m) + a[1] * T3(1 + k, 1 + l, m) + a[2] * T3(k, 2 + l, m) + a[3] * T3(k, 1 + l, 1 + m) + a[4] * h2;
+    double h3 = 0.5 * (T3(2 + k, l, 1 + m) + T3(k, 2 + l, 1 + m) + T3(k, l, 3 + m) + T3(k, l, 1 + m) * e2);
+    M[3] = a[0] * T3(k, l, 1 + m) + a[1] * T3(1 + k, l, 1 + m) + a[2] * T3(k, 1 + l, 1 + m) + a[3] * T3(k, l, 2 + m) + a[4] * h3;
+    M[4] = 0.25 * a[4] *
+               (T3(4 + k, l, m) + T3(k, 4 + l, m) + T3(k, l, 4 + m) + T3(k, l, m) * e4 + 2 * T3(2 + k, 2 + l, m) +
+                2 * T3(2 + k, l, 2 + m) + 2 * T3(2 + k, l, m) * e2 + 2 * T3(k, 2 + l, 2 + m) + 2 * T3(k, 2 + l, m) * e2 +
+                2 * T3(k, l, 2 + m) * e2) +
+           a[0] * h0 + a[1] * h1 + a[2] * h2 + a[3] * h3;
+#undef T3
+}
+
+/* GKMflux.cpp:283-297 -- micro-slope a from macro-slope b; a[4] scaled by 1/Pr */
+static void slope(const double b[5], const double U[3], double lam, const orc_gas *g, double a[5])
+{
+    const int K = g->K;
+    double R2 = b[1] - U[0] * b[0];
+    double R3 = b[2] - U[1] * b[0];
+    double R4 = b[3] - U[2] * b[0];
+    double R5 = 2 * b[4] - b[0] * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2] + (K + 3) / (2 * lam));
+    a[4] = (1 / g->prn) * (R5 - 2 * U[0] * R2 - 2 * U[1] * R3 - 2 * U[2] * R4) * (4 * lam * lam) / (K + 3);
+    a[3] = 2 * lam * R4 - U[2] * a[4];
+    a[2] = 2 * lam * R3 - U[1] * a[4];
+    a[1] = 2 * lam * R2 - U[0] * a[4];
+    a[0] = b[0] - a[1] * U[0] - a[2] * U[1] - a[3] * U[2] -
+           .5 * a[4] * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2] + (K + 3) / (2 * lam));
+}
+
+/* GKMflux.cpp:11-229 */
+void orc_flux(const double WL[5], const double WR[5], const double Wl[5], const double Wr[5],
+              const double DWxl[5], const double DWxr[5], const double DWyl[5],
+              const double DWyr[5], const double DWzl[5], const double DWzr[5],
+              double tau, double dt, double dx, const orc_gas *g, double F[5])
+{
+    gstate L, Rs, E;
+    gstate_init(&L, Wl, g, 1, 0);  /* left state: positive half-space of u  */
+    gstate_init(&Rs, Wr, g, 0, 1); /* right state: negative half-space of u */
+
+    /* GKMflux.cpp:73-80: equilibrium state at the interface */
+    double Mpl[5], Mnr[5], We[5];
+    mom(L.hp, &L, 0, 0, 0, Mpl);
+    mom(Rs.hn, &Rs, 0, 0, 0, Mnr);
+    for (int q = 0; q < 5; q++) We[q] = L.den * Mpl[q] + Rs.den * Mnr[q];
+    gstate_init(&E, We, g, 1, 1);
+
+    /* GKMflux.cpp:107-125: slopes, phase I */
+    double bxl[5], bxr[5], byl[5], byr[5], bzl[5], bzr[5], bxel[5], bxer[5];
+    for (int q = 0; q < 5; q++) {
+        bxl[q] = DWxl[q] / L.den;
+        bxr[q] = DWxr[q] / Rs.den;
+        byl[q] = DWyl[q] / L.den;
+        byr[q] = DWyr[q] / Rs.den;
+        bzl[q] = DWzl[q] / L.den;
+        bzr[q] = DWzr[q] / Rs.den;
+        bxel[q] = 2 * (We[q] - WL[q]) / (E.den * dx);
+        bxer[q] = 2 * (WR[q] - We[q]) / (E.den * dx);
+    }
+    double axl[5], axr[5], ayl[5], ayr[5], azl[5], azr[5], axel[5], axer[5];
+    slope(bxl, L.U, L.lam, g, axl);
+    slope(bxr, Rs.U, Rs.lam, g, axr);
+    slope(byl, L.U, L.lam, g, ayl);
+    slope(byr, Rs.U, Rs.lam, g, ayr);
+    slope(bzl, L.U, L.lam, g, azl);
+    slope(bzr, Rs.U, Rs.lam, g, azr);
+    slope(bxel, E.U, E.lam, g, axel);
+    slope(bxer, E.U, E.lam, g, axer);
+
+    /* GKMflux.cpp:130-151: slopes, phase II */
+    double m1[5], m2[5], m3[5], m4[5], m5[5], m6[5], m7[5], m8[5], m9[5], m10[5];
+    mom_a(L.hp, &L, 0, 0, 0, ayl, m1);
+    mom_a(Rs.hn, &Rs, 0, 0, 0, ayr, m2);
+    mom_a(L.hp, &L, 0, 0, 0, azl, m3);
+    mom_a(Rs.hn, &Rs, 0, 0, 0, azr, m4);
+    mom_a(L.fu, &L, 1, 0, 0, axl, m5);
+    mom_a(Rs.fu, &Rs, 1, 0, 0, axr, m6);
+    mom_a(L.fu, &L, 0, 1, 0, ayl, m7);
+    mom_a(Rs.fu, &Rs, 0, 1, 0, ayr, m8);
+    mom_a(L.fu, &L, 0, 0, 1, azl, m9);
+    mom_a(Rs.fu, &Rs, 0, 0, 1, azr, m10);
+    double bye[5], bze[5], Bl[5], Br[5];
+    for (int q = 0; q < 5; q++) {
+        bye[q] = m1[q] + m2[q];
+        bze[q] = m3[q] + m4[q];
+        Bl[q] = -m5[q] - m7[q] - m9[q];
+        Br[q] = -m6[q] - m8[q] - m10[q];
+    }
+    double aye[5], aze[5], Al[5], Ar[5];
+    slope(bye, E.U, E.lam, g, aye);
+    slope(bze, E.U, E.lam, g, aze);
+    slope(Bl, L.U, L.lam, g, Al);
+    slope(Br, Rs.U, Rs.lam, g, Ar);
+
+    /* GKMflux.cpp:156-157: artificial collision time from the pressure jump */
+    double w = fabs(L.den / L.lam - Rs.den / Rs.lam) / (fabs(L.den / L.lam + Rs.den / Rs.lam));
+    tau = tau + dt * w;
+
+    /* GKMflux.cpp:162-191: slopes, phase III (time derivative A_) */
+    const double ex = exp(-dt / tau);
+    double gm0 = dt - tau * (1 - ex);
+    double gm1 = -(1 - ex) / gm0;
+    double gm2 = (-dt + 2 * tau * (1 - ex) - dt * ex) / gm0;
+    double gm3 = -gm1;
+    double gm4 = (dt * ex - tau * (1 - ex)) / gm0;
+    double gm5 = tau * gm3;
+
+    double e000[5], e100axl[5], e100axr[5], e010[5], e001[5], l100[5], r100[5], l010[5], r010[5], l001[5],
+        r001[5], l000A[5], r000A[5];
+    mom(E.fu, &E, 0, 0, 0, e000);
+    mom_a(E.hp, &E, 1, 0, 0, axel, e100axl);
+    mom_a(E.hn, &E, 1, 0, 0, axer, e100axr);
+    mom_a(E.fu, &E, 0, 1, 0, aye, e010);
+    mom_a(E.fu, &E, 0, 0, 1, aze, e001);
+    mom_a(L.hp, &L, 1, 0, 0, axl, l100);
+    mom_a(Rs.hn, &Rs, 1, 0, 0, axr, r100);
+    mom_a(L.hp, &L, 0, 1, 0, ayl, l010);
+    mom_a(Rs.hn, &Rs, 0, 1, 0, ayr, r010);
+    mom_a(L.hp, &L, 0, 0, 1, azl, l001);
+    mom_a(Rs.hn, &Rs, 0, 0, 1, azr, r001);
+    mom_a(L.hp, &L, 0, 0, 0, Al, l000A);
+    mom_a(Rs.hn, &Rs, 0, 0, 0, Ar, r000A);
+    double Be[5];
+    for (int q = 0; q < 5; q++) {
+        Be[q] = gm1 * E.den * e000[q] + gm2 * E.den * (e100axl[q] + e100axr[q] + e010[q] + e001[q]) +
+                gm3 * (L.den * Mpl[q] + Rs.den * Mnr[q]) +
+                (gm4 + gm5) * (L.den * (l100[q] + l010[q] + l001[q]) + Rs.den * (r100[q] + r010[q] + r001[q])) +
+                gm5 * (L.den * l000A[q] + Rs.den * r000A[q]);
+        Be[q] = Be[q] / E.den;
+    }
+    double Ae[5];
+    slope(Be, E.U, E.lam, g, Ae);
+
+    /* GKMflux.cpp:197-228: time-integrated flux over [0, dt] */
+    double P = -tau * (ex - 1);
+    double Q = -tau * dt * ex - tau * tau * (ex - 1);
+    double p0 = (dt - P);
+    double p1 = 0.5 * dt * dt - tau * (p0);
+    double p2 = -tau * (p0) + Q;
+    double p3 = P;
+    double p4 = -Q - tau * P;
+    double p5 = -tau * P;
+
+    double pl100[5], nr100[5], e100[5], e100A[5], e200l[5], e200r[5], e110[5], e101[5], l200[5], r200[5],
+        l110[5], r110[5], l101[5], r101[5], l100A[5], r100A[5];
+    mom(L.hp, &L, 1, 0, 0, pl100);
+    mom(Rs.hn, &Rs, 1, 0, 0, nr100);
+    mom(E.fu, &E, 1, 0, 0, e100);
+    mom_a(E.fu, &E, 1, 0, 0, Ae, e100A);
+    mom_a(E.hp, &E, 2, 0, 0, axel, e200l);
+    mom_a(E.hn, &E, 2, 0, 0, axer, e200r);
+    mom_a(E.fu, &E, 1, 1, 0, aye, e110);
+    mom_a(E.fu, &E, 1, 0, 1, aze, e101);
+    mom_a(L.hp, &L, 2, 0, 0, axl, l200);
+    mom_a(Rs.hn, &Rs, 2, 0, 0, axr, r200);
+    mom_a(L.hp, &L, 1, 1, 0, ayl, l110);
+    mom_a(Rs.hn, &Rs, 1, 1, 0, ayr, r110);
+    mom_a(L.hp, &L, 1, 0, 1, azl, l101);
+    mom_a(Rs.hn, &Rs, 1, 0, 1, azr, r101);
+    mom_a(L.hp, &L, 1, 0, 0, Al, l100A);
+    mom_a(Rs.hn, &Rs, 1, 0, 0, Ar, r100A);
+    for (int q = 0; q < 5; q++) {
+        F[q] = p0 * E.den * e100[q] + p1 * E.den * e100A[q] +
+               p2 * E.den * (e200l[q] + e200r[q] + e110[q] + e101[q]) + p3 * (L.den * pl100[q] + Rs.den * nr100[q]) +
+               p4 * (L.den * (l200[q] + l110[q] + l101[q]) + Rs.den * (r200[q] + r110[q] + r101[q])) +
+               p5 * (L.den * l100A[q] + Rs.den * r100A[q]);
+    }
+}
+
+void orc_flux_batch(long n, const double *in, const double *tau, double dt, const orc_gas *g, double *F)
+{
+    for (long i = 0; i < n; i++) {
+        const double *v = in + 50 * i;
+        orc_flux(v, v + 5, v + 10, v + 15, v + 20, v + 25, v + 30, v + 35, v + 40, v + 45, tau[i], dt, g->dx, g,
+                 F + 5 * i);
+    }
+}
+
+/* Collision time of the interface's left cell, Sutherland's law:
+ * update.cpp:276-278 with the global c2p() of update.cpp:844-853.             */
+double orc_tau(const double WL[5], const orc_gas *g)
+{
+    double deni = WL[0];
+    double ui = WL[1] / deni, vi = WL[2] / deni, wi = WL[3] / deni;
+    double Vsqr = (ui * ui + vi * vi + wi * wi);
+    double Pi = (deni * (g->gam - 1)) * (WL[4] / deni - 0.5 * Vsqr);
+    double Ti = Pi / (g->R * deni);
+    double mu = g->mu0 * pow((Ti / g->T0), 1.5) * (g->T0 + 110.4) / (Ti + 110.4);
+    return mu / Pi;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Periodic halo: the single-domain result of HaloTransfer (update.cpp:855-   */
+/* 1029) with procDim 1x1x1, i.e. ghost(i) = cell(((i-3) mod nc) + 3) on each */
+/* axis, corners included (the Y -> X -> Z order fills them).                 */
+/* ------------------------------------------------------------------------- */
+void orc_halo_fill(int nc, double *W)
+{
+    const int nt = nc + 6;
+    const long Nt = (long)nt * nt * nt;
+    for (int q = 0; q < 5; q++) {
+        double *w = W + q * Nt;
+        for (int j = 0; j < nt; j++)
+            for (int i = 0; i < nt; i++)
+                for (int k = 0; k < nt; k++) {
+                    int jj = ((j - 3) % nc + nc) % nc + 3;
+                    int ii = ((i - 3) % nc + nc) % nc + 3;
+                    int kk = ((k - 3) % nc + nc) % nc + 3;
+                    if (jj != j || ii != i || kk != k) w[pidx(nt, j, i, k)] = w[pidx(nt, jj, ii, kk)];
+                }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* update.cpp:214-503 -- derivsX / derivsY / derivsZ.                         */
+/* dir 0 = x, 1 = y, 2 = z.  F is stored at the left cell of face i+1/2.       */
+/* ------------------------------------------------------------------------- */
+static void sweep(int nc, const double *W, double dt, const orc_gas *g, int dir, int nthreads, double *Fo)
+{
+    const int nt = nc + 6;
+    const long Nt = (long)nt * nt * nt;
+    const double dx = g->dx;
+    /* strides (in the padded array) of the sweep axis and of the two tangential
+     * neighbour lines the reference calls N/S and F/B.                          */
+    long sj = (long)nt * nt, si = nt, sk = 1;
+    long s_n, s_nN, s_nF; /* normal stride; the "N" offset; the "F" offset       */
+    double ysign = 0.5;
+    int jlo = 3, ilo = 3, klo = 3;
+    if (dir == 0) {        /* update.cpp:220-253: N = j+1, S = j-1, F = k+1, B = k-1 */
+        s_n = si; s_nN = sj; s_nF = sk; ilo = 2;
+    } else if (dir == 1) { /* update.cpp:293-326: N = i-1, S = i+1, F = k+1, B = k-1; DWy sign -0.5 */
+        s_n = sj; s_nN = -si; s_nF = sk; jlo = 2; ysign = -0.5;
+    } else {               /* update.cpp:402-435: N = i+1, S = i-1, F = j+1, B = j-1 */
+        s_n = sk; s_nN = si; s_nF = sj; klo = 2;
+    }
+    /* component swap of the rotated frame: Y 1<->2 (update.cpp:361-398), Z 1<->3 (update.cpp:461-497) */
+    const int swp = dir == 0 ? 1 : (dir == 1 ? 2 : 3);
+
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1) if (nthreads > 1)
+    for (int j = jlo; j < nt - 3; j++)
+        for (int i = ilo; i < nt - 3; i++)
+            for (int k = klo; k < nt - 3; k++) {
+                const long c = pidx(nt, j, i, k);
+                const long lines[5] = {0, s_nN, -s_nN, s_nF, -s_nF}; /* centre, N, S, F, B */
+                double Wlr[5][2][5];
+                double S[6 * 5];
+                for (int ln = 0; ln < 5; ln++) {
+                    for (int s = 0; s < 6; s++)
+                        for (int q = 0; q < 5; q++) S[s * 5 + q] = W[q * Nt + c + lines[ln] + (s - 2) * s_n];
+                    orc_weno(S, g, Wlr[ln][0], Wlr[ln][1]);
+                }
+                double WL[5], WR[5], Wl[5], Wr[5], Dxl[5], Dxr[5], Dyl[5], Dyr[5], Dzl[5], Dzr[5];
+                for (int q = 0; q < 5; q++) {
+                    WL[q] = W[q * Nt + c];
+                    WR[q] = W[q * Nt + c + s_n];
+                    Wl[q] = Wlr[0][0][q];
+                    Wr[q] = Wlr[0][1][q];
+                    Dxl[q] = 2 * (Wl[q] - WL[q]) / dx;
+                    Dxr[q] = 2 * (WR[q] - Wr[q]) / dx;
+                    Dyl[q] = ysign * (Wlr[1][0][q] - Wlr[2][0][q]) / dx;
+                    Dyr[q] = ysign * (Wlr[1][1][q] - Wlr[2][1][q]) / dx;
+                    Dzl[q] = 0.5 * (Wlr[3][0][q] - Wlr[4][0][q]) / dx;
+                    Dzr[q] = 0.5 * (Wlr[3][1][q] - Wlr[4][1][q]) / dx;
+                }
+                double tau = orc_tau(WL, g);
+                if (swp != 1) {
+                    double *vs[10] = {WL, WR, Wl, Wr, Dxl, Dxr, Dyl, Dyr, Dzl, Dzr};
+                    for (int v = 0; v < 10; v++) {
+                        double t = vs[v][swp];
+                        vs[v][swp] = vs[v][1];
+                        vs[v][1] = t;
+                    }
+                }
+                double F[5];
+                orc_flux(WL, WR, Wl, Wr, Dxl, Dxr, Dyl, Dyr, Dzl, Dzr, tau, dt, dx, g, F);
+                if (swp != 1) {
+                    double t = F[swp];
+                    F[swp] = F[1];
+                    F[1] = t;
+                }
+                for (int q = 0; q < 5; q++) Fo[q * Nt + c] = F[q];
+            }
+}
+
+void orc_fluxes(int nc, const double *W, double dt, const orc_gas *g, int nthreads, double *Fx, double *Fy,
+                double *Fz)
+{
+    sweep(nc, W, dt, g, 0, nthreads, Fx);
+    sweep(nc, W, dt, g, 1, nthreads, Fy);
+    sweep(nc, W, dt, g, 2, nthreads, Fz);
+}
+
+/* update.cpp:505-516 */
+void orc_update(int nc, double *W, const double *Fx, const double *Fy, const double *Fz, double dx)
+{
+    const int nt = nc + 6;
+    const long Nt = (long)nt * nt * nt;
+    const long sj = (long)nt * nt, si = nt;
+    for (int j = 3; j < nt - 3; j++)
+        for (int i = 3; i < nt - 3; i++)
+            for (int k = 3; k < nt - 3; k++)
+                for (int q = 0; q < 5; q++) {
+                    const long c = q * Nt + pidx(nt, j, i, k);
+                    W[c] -= (1 / dx) * (Fx[c] - Fx[c - si] + Fy[c] - Fy[c - sj] + Fz[c] - Fz[c - 1]);
+                }
+}
+
+/* evolveEulerian (update.cpp:206-212) followed by the periodic halo fill */
+void orc_eulerian_step(int nc, double *W, double dt, const orc_gas *g, int nthreads)
+{
+    const int nt = nc + 6;
+    const long Nt5 = 5L * nt * nt * nt;
+    double *F = (double *)calloc(3 * Nt5, sizeof(double));
+    orc_fluxes(nc, W, dt, g, nthreads, F, F + Nt5, F + 2 * Nt5);
+    orc_update(nc, W, F, F + Nt5, F + 2 * Nt5, g->dx);
+    orc_halo_fill(nc, W);
+    free(F);
+}
+
+/* update.cpp:1031-1054 -- dt = cfl dx / max(|U| + c) over the interior */
+double orc_cfl_dt(int nc, const double *W, double cfl, const orc_gas *g)
+{
+    const int nt = nc + 6;
+    const long Nt = (long)nt * nt * nt;
+    double vmax = 0.0;
+    for (int j = 3; j < nt - 3; j++)
+        for (int i = 3; i < nt - 3; i++)
+            for (int k = 3; k < nt - 3; k++) {
+                const long c = pidx(nt, j, i, k);
+                double den = W[c], u = W[Nt + c] / den, v = W[2 * Nt + c] / den, w = W[3 * Nt + c] / den;
+                double Vsqr = (u * u + v * v + w * w);
+                double P = (den * (g->gam - 1)) * (W[4 * Nt + c] / den - 0.5 * Vsqr);
+                double val = sqrt(Vsqr) + sqrt(g->gam * P / den);
+                if (vmax < val) vmax = val;
+            }
+    return cfl * g->dx / vmax;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Natural tricubic B-spline (einspline's UBspline_3d_d with NATURAL ends,    */
+/* built by get_bsplinep, Interpolation.cpp:3-31).  Grid per axis: nt knots    */
+/* from start = -3 dx + dx/2 with spacing delta = (end-start)/(nt-1).  The     */
+/* coefficients c[0..nt+1] of a line satisfy (c_i + 4 c_{i+1} + c_{i+2})/6 =   */
+/* f_i and c_0 - 2c_1 + c_2 = 0 = c_{nt-1} - 2c_nt + c_{nt+1} (f'' = 0 ends).   */
+/* ------------------------------------------------------------------------- */
+static void natural_line(int M, const double *f, long fs, double *c, long cs, double *work)
+{
+    /* c1 = f0 and c_M = f_{M-1} follow from the end rows; the inner
+     * unknowns c_2..c_{M-1} solve a [1 4 1] system (Thomas algorithm).        */
+    const int n = M - 2;
+    double *cp = work, *dp = work + M;
+    double c1 = f[0], cM = f[(long)(M - 1) * fs];
+    for (int r = 0; r < n; r++) {
+        double rhs = 6.0 * f[(long)(r + 1) * fs];
+        if (r == 0) rhs -= c1;
+        if (r == n - 1) rhs -= cM;
+        double diag = 4.0 - (r > 0 ? cp[r - 1] : 0.0);
+        cp[r] = 1.0 / diag;
+        dp[r] = (rhs - (r > 0 ? dp[r - 1] : 0.0)) / diag;
+    }
+    for (int r = n - 2; r >= 0; r--) dp[r] -= cp[r] * dp[r + 1];
+    c[1 * cs] = c1;
+    for (int r = 0; r < n; r++) c[(long)(r + 2) * cs] = dp[r];
+    c[(long)M * cs] = cM;
+    c[0] = 2 * c1 - c[2 * cs];
+    c[(long)(M + 1) * cs] = 2 * cM - c[(long)(M - 1) * cs];
+}
+
+void orc_spline_coefs(int nc, const double *f, double *coefs)
+{
+    const int M = nc + 6, N = M + 2;
+    double *work = (double *)malloc(sizeof(double) * 2 * N);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)M * M * N);
+    double *tmp2 = (double *)malloc(sizeof(double) * (size_t)M * N * N);
+    /* z lines: f (M,M,M) -> tmp (M,M,N) */
+    for (int a = 0; a < M; a++)
+        for (int b = 0; b < M; b++) natural_line(M, f + ((long)a * M + b) * M, 1, tmp + ((long)a * M + b) * N, 1, work);
+    /* x lines: tmp (M,M,N) -> tmp2 (M,N,N) */
+    for (int a = 0; a < M; a++)
+        for (int c = 0; c < N; c++)
+            natural_line(M, tmp + (long)a * M * N + c, N, tmp2 + (long)a * N * N + c, N, work);
+    /* y lines: tmp2 (M,N,N) -> coefs (N,N,N) */
+    for (int b = 0; b < N; b++)
+        for (int c = 0; c < N; c++) natural_line(M, tmp2 + (long)b * N + c, (long)N * N, coefs + (long)b * N + c, (long)N * N, work);
+    free(work);
+    free(tmp);
+    free(tmp2);
+}
+
+/* cubic B-spline weights, their first and second t-derivatives */
+static void bweights(double t, double w[4], double dw[4], double d2w[4])
+{
+    double t2 = t * t, t3 = t2 * t;
+    w[0] = (-t3 + 3 * t2 - 3 * t + 1) / 6.0;
+    w[1] = (3 * t3 - 6 * t2 + 4) / 6.0;
+    w[2] = (-3 * t3 + 3 * t2 + 3 * t + 1) / 6.0;
+    w[3] = t3 / 6.0;
+    dw[0] = -0.5 * t2 + t - 0.5;
+    dw[1] = 1.5 * t2 - 2 * t;
+    dw[2] = -1.5 * t2 + t + 0.5;
+    dw[3] = 0.5 * t2;
+    d2w[0] = 1 - t;
+    d2w[1] = 3 * t - 2;
+    d2w[2] = 1 - 3 * t;
+    d2w[3] = t;
+}
+
+static void grid_geom(int nc, double dx, double *start, double *dinv)
+{
+    const int nt = nc + 6;
+    double s = (0 - 3) * dx + 0.5 * dx;     /* Interpolation.cpp:7 with sta = 0 */
+    double e = s + (nt - 1) * dx;           /* Interpolation.cpp:8             */
+    double delta = (e - s) / (double)(nt - 1);
+    *start = s;
+    *dinv = 1.0 / delta;
+}
+
+/* value, gradient (d/dy, d/dx, d/dz) and Hessian (row-major over y,x,z) */
+void orc_spline_eval(int nc, const double *coefs, double dx, double y, double x, double z, double *val,
+                     double grad[3], double hess[9])
+{
+    const int N = nc + 8;
+    double start, dinv;
+    grid_geom(nc, dx, &start, &dinv);
+    double p[3] = {y, x, z}, w[3][4], dw[3][4], d2w[3][4];
+    int ix[3];
+    for (int a = 0; a < 3; a++) {
+        double u = (p[a] - start) * dinv, ip;
+        double t = modf(u, &ip);
+        ix[a] = (int)ip;
+        bweights(t, w[a], dw[a], d2w[a]);
+    }
+    double v = 0, g[3] = {0, 0, 0}, h[9] = {0};
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++)
+            for (int c = 0; c < 4; c++) {
+                double P = coefs[((long)(ix[0] + a) * N + (ix[1] + b)) * N + (ix[2] + c)];
+                v += w[0][a] * w[1][b] * w[2][c] * P;
+                g[0] += dw[0][a] * w[1][b] * w[2][c] * P;
+                g[1] += w[0][a] * dw[1][b] * w[2][c] * P;
+                g[2] += w[0][a] * w[1][b] * dw[2][c] * P;
+                h[0] += d2w[0][a] * w[1][b] * w[2][c] * P;
+                h[1] += dw[0][a] * dw[1][b] * w[2][c] * P;
+                h[2] += dw[0][a] * w[1][b] * dw[2][c] * P;
+                h[4] += w[0][a] * d2w[1][b] * w[2][c] * P;
+                h[5] += w[0][a] * dw[1][b] * dw[2][c] * P;
+                h[8] += w[0][a] * w[1][b] * d2w[2][c] * P;
+            }
+    if (val) *val = v;
+    if (grad)
+        for (int a = 0; a < 3; a++) grad[a] = g[a] * dinv;
+    if (hess) {
+        double d2 = dinv * dinv;
+        hess[0] = h[0] * d2;
+        hess[1] = hess[3] = h[1] * d2;
+        hess[2] = hess[6] = h[2] * d2;
+        hess[4] = h[4] * d2;
+        hess[5] = hess[7] = h[5] * d2;
+        hess[8] = h[8] * d2;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Particles                                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* generateSplines(W, splinepn, u, v, w), update.cpp:1235-1247 */
+void orc_velocity_coefs(int nc, const double *W, double *cu, double *cv, double *cw)
+{
+    const int nt = nc + 6;
+    const long Nt = (long)nt * nt * nt;
+    double *f = (double *)malloc(sizeof(double) * Nt);
+    double *out[3] = {cu, cv, cw};
+    for (int q = 1; q <= 3; q++) {
+        for (long c = 0; c < Nt; c++) f[c] = W[q * Nt + c] / W[c];
+        orc_spline_coefs(nc, f, out[q - 1]);
+    }
+    free(f);
+}
+
+/* Particle::getParticleProp velocity subset, ParticleTrack.cpp:99-113:
+ * pos = (y, x, z); U = (u, v, w); gU/gV/gW = gradients over (y, x, z).       */
+void orc_particle_sample(int nc, const double *cu, const double *cv, const double *cw, double dx, long np,
+                         const double *pos, double *U, double *gU, double *gV, double *gW)
+{
+    for (long p = 0; p < np; p++) {
+        const double *x = pos + 3 * p;
+        orc_spline_eval(nc, cu, dx, x[0], x[1], x[2], U + 3 * p + 0, gU ? gU + 3 * p : NULL, NULL);
+        orc_spline_eval(nc, cv, dx, x[0], x[1], x[2], U + 3 * p + 1, gV ? gV + 3 * p : NULL, NULL);
+        orc_spline_eval(nc, cw, dx, x[0], x[1], x[2], U + 3 * p + 2, gW ? gW + 3 * p : NULL, NULL);
+    }
+}
+
+/* Particle::updatePosition, ParticleTrack.cpp:48-73 -- Heun RK2; predictor not
+ * wrapped, corrector wrapped into [0, 2 pi].                                  */
+void orc_particle_heun(int nc, const double *cu, const double *cv, const double *cw, double dx, long np,
+                       double *pos, const double *U, double dt)
+{
+    const double twopi = 2 * ORC_PI;
+    for (long p = 0; p < np; p++) {
+        double *x = pos + 3 * p;
+        const double *u = U + 3 * p;
+        double y1 = x[0] + u[1] * dt, x1 = x[1] + u[0] * dt, z1 = x[2] + u[2] * dt;
+        double U2[3];
+        orc_spline_eval(nc, cu, dx, y1, x1, z1, &U2[0], NULL, NULL);
+        orc_spline_eval(nc, cv, dx, y1, x1, z1, &U2[1], NULL, NULL);
+        orc_spline_eval(nc, cw, dx, y1, x1, z1, &U2[2], NULL, NULL);
+        x[0] += 0.5 * (u[1] + U2[1]) * dt;
+        x[1] += 0.5 * (u[0] + U2[0]) * dt;
+        x[2] += 0.5 * (u[2] + U2[2]) * dt;
+        for (int q = 0; q < 3; q++) {
+            if (x[q] > twopi) x[q] = fmod(x[q], twopi);
+            if (x[q] < 0) x[q] = twopi - fmod(-x[q], twopi);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* PrintStats, update.cpp:688-842 (plus enstrophy <|curl u|^2>, which the      */
+/* reference does not print).  Velocity gradients from the spline at cells.   */
+/* ------------------------------------------------------------------------- */
+void orc_field_stats(int nc, const double *W, const double *cu, const double *cv, const double *cw, const orc_gas *g,
+                     double p0, double Mt, double den0, orc_stats_t *out)
+{
+    const int nt = nc + 6;
+    const long Nt = (long)nt * nt * nt;
+    const double Nc = (double)nc * nc * nc;
+    const double dx = g->dx;
+    double tke = 0, diss = 0, sij = 0, ens = 0, d2[3] = {0}, d3[3] = {0}, d4[3] = {0};
+    double pm = 0, tm = 0, vm = 0;
+    for (int j = 3; j < nt - 3; j++)
+        for (int i = 3; i < nt - 3; i++)
+            for (int k = 3; k < nt - 3; k++) {
+                const long c = pidx(nt, j, i, k);
+                double den = W[c], u = W[Nt + c] / den, v = W[2 * Nt + c] / den, w = W[3 * Nt + c] / den;
+                double Vsqr = u * u + v * v + w * w;
+                double P = (den * (g->gam - 1)) * (W[4 * Nt + c] / den - 0.5 * Vsqr);
+                double T = P / (g->R * den);
+                pm += P;
+                tm += T;
+                vm += 1 / den;
+                double y = (j - 3) * dx + 0.5 * dx, x = (i - 3) * dx + 0.5 * dx, z = (k - 3) * dx + 0.5 * dx;
+                double gu[3], gv[3], gw[3], ui, vi, wi;
+                orc_spline_eval(nc, cu, dx, y, x, z, &ui, gu, NULL);
+                orc_spline_eval(nc, cv, dx, y, x, z, &vi, gv, NULL);
+                orc_spline_eval(nc, cw, dx, y, x, z, &wi, gw, NULL);
+                double dudx = gu[1], dudy = gu[0], dudz = gu[2];
+                double dvdx = gv[1], dvdy = gv[0], dvdz = gv[2];
+                double dwdx = gw[1], dwdy = gw[0], dwdz = gw[2];
+                double mu = g->mu0 * pow((T / g->T0), 1.5) * (g->T0 + 110.4) / (T + 110.4);
+                diss += mu * (dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
+                              dwdx * dwdx + dwdy * dwdy + dwdz * dwdz);
+                /* density at the knot: the spline interpolates the cell value */
+                tke += den * (ui * ui + vi * vi + wi * wi);
+                sij += (dudx * dudx + 0.5 * sq(dudy + dvdx) + 0.5 * sq(dudz + dwdx) + 0.5 * sq(dwdy + dvdz) +
+                        dvdy * dvdy + dwdz * dwdz);
+                ens += sq(dwdy - dvdz) + sq(dudz - dwdx) + sq(dvdx - dudy);
+                double dd[3] = {dudx, dvdy, dwdz};
+                for (int a = 0; a < 3; a++) {
+                    d2[a] += dd[a] * dd[a];
+                    d3[a] += dd[a] * dd[a] * dd[a];
+                    d4[a] += dd[a] * dd[a] * dd[a] * dd[a];
+                }
+            }
+    out->tke = tke / Nc;
+    out->diss = diss / Nc;
+    out->sijsij = sij / Nc;
+    out->ens = ens / Nc;
+    for (int a = 0; a < 3; a++) {
+        double m2 = d2[a] / Nc, m3 = d3[a] / Nc, m4 = d4[a] / Nc;
+        out->skew[a] = m3 / pow(m2, 1.5);
+        out->kurt[a] = m4 / pow(m2, 2);
+    }
+    pm /= Nc;
+    tm /= Nc;
+    vm /= Nc;
+    double pr = 0, tr = 0, vr = 0;
+    for (int j = 3; j < nt - 3; j++)
+        for (int i = 3; i < nt - 3; i++)
+            for (int k = 3; k < nt - 3; k++) {
+                const long c = pidx(nt, j, i, k);
+                double den = W[c], u = W[Nt + c] / den, v = W[2 * Nt + c] / den, w = W[3 * Nt + c] / den;
+                double Vsqr = u * u + v * v + w * w;
+                double P = (den * (g->gam - 1)) * (W[4 * Nt + c] / den - 0.5 * Vsqr);
+                double T = P / (g->R * den);
+                pr += sq(pm - P);
+                tr += sq(tm - T);
+                vr += sq(vm - 1 / den);
+            }
+    out->pmean = pm;
+    out->tmean = tm;
+    out->vmean = vm;
+    out->prms = sqrt(pr / Nc) / (g->gam * p0 * Mt * Mt);
+    out->trms = sqrt(tr / Nc) / ((g->gam - 1) * g->T0 * Mt * Mt);
+    out->vrms = sqrt(vr / Nc) / (Mt * Mt / den0);
+}
+
+/* GetParticleStats, update.cpp:1111-1162 (no density weight, / numparticles) */
+void orc_particle_stats(long np, long np_total, const double *U, const double *gU, const double *gV, const double *gW,
+                        orc_pstats_t *out)
+{
+    double tke = 0, sij = 0, u2 = 0, u3 = 0, w2 = 0, w3 = 0;
+    for (long p = 0; p < np; p++) {
+        const double *u = U + 3 * p;
+        tke += (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double dudx = gU[3 * p + 1], dvdx = gV[3 * p + 1], dwdx = gW[3 * p + 1];
+        double dudy = gU[3 * p + 0], dvdy = gV[3 * p + 0], dwdy = gW[3 * p + 0];
+        double dudz = gU[3 * p + 2], dvdz = gV[3 * p + 2], dwdz = gW[3 * p + 2];
+        sij += (dudx * dudx + 0.5 * sq(dudy + dvdx) + 0.5 * sq(dudz + dwdx) + 0.5 * sq(dwdy + dvdz) + dvdy * dvdy +
+                dwdz * dwdz);
+        u2 += dudx * dudx;
+        u3 += dudx * dudx * dudx;
+        w2 += dwdz * dwdz;
+        w3 += dwdz * dwdz * dwdz;
+    }
+    double n = (double)np_total;
+    out->tke = tke / n;
+    out->sijsij = sij / n;
+    out->skew_u = (u3 / n) / pow(u2 / n, 1.5);
+    out->skew_w = (w3 / n) / pow(w2 / n, 1.5);
+}

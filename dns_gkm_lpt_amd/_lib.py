@@ -1,0 +1,219 @@
+"""ctypes binding of libdnsgkm.so (the C ABI of include/dnsgkm.h).
+
+The HIP library is the only compute path: if libdnsgkm.so is missing this
+module raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdnsgkm.so")
+
+GKM_FP64 = 0
+GKM_MIXED = 1
+ERRORS = {1: "GKM_EINVAL", 2: "GKM_EHIP", 3: "GKM_ENOMEM", 4: "GKM_ESTATE", 5: "GKM_ENOTREADY"}
+
+# every entry point include/dnsgkm.h declares
+EXPORTS = [
+    "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_upload_state", "gkm_download_state",
+    "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
+    "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
+    "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_coupled_step", "lpt_heun",
+]
+
+
+class Config(C.Structure):
+    _fields_ = [("nc", C.c_int), ("precision", C.c_int), ("gam", C.c_double), ("rho0", C.c_double),
+                ("Mt", C.c_double), ("Re", C.c_double), ("mu0", C.c_double), ("p0", C.c_double),
+                ("t0", C.c_double), ("cfl", C.c_double), ("prandtl", C.c_double), ("device", C.c_int),
+                ("check_state", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("tke", C.c_double), ("diss", C.c_double), ("sijsij", C.c_double), ("skew", C.c_double * 3),
+                ("kurt", C.c_double * 3), ("prms", C.c_double), ("trms", C.c_double), ("vrms", C.c_double),
+                ("pmean", C.c_double), ("tmean", C.c_double), ("vmean", C.c_double),
+                ("enstrophy", C.c_double)]
+
+
+class PStats(C.Structure):
+    _fields_ = [("tke", C.c_double), ("sijsij", C.c_double), ("skew_u", C.c_double), ("skew_w", C.c_double)]
+
+
+class GkmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run dns_gkm_lpt_amd.build.build() (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.gkm_create.argtypes = [C.POINTER(Config), C.POINTER(P)]
+        L.gkm_destroy.argtypes = [P]
+        L.gkm_destroy.restype = None
+        L.gkm_last_error.argtypes = [P]
+        L.gkm_last_error.restype = C.c_char_p
+        L.gkm_get_K.argtypes = [P]
+        L.gkm_upload_state_packed.argtypes = [P, _dp]
+        L.gkm_download_state_packed.argtypes = [P, _dp]
+        L.gkm_upload_state.argtypes = [P, C.POINTER(C.c_void_p)]
+        L.gkm_download_state.argtypes = [P, C.POINTER(C.c_void_p)]
+        L.gkm_step.argtypes = [P, C.c_double]
+        L.gkm_cfl_dt.argtypes = [P, C.POINTER(C.c_double)]
+        L.gkm_run.argtypes = [P, C.c_int, C.c_void_p]
+        L.gkm_sync.argtypes = [P]
+        L.gkm_stats_compute.argtypes = [P, C.POINTER(Stats)]
+        L.gkm_timing_enable.argtypes = [P, C.c_int]
+        L.gkm_timing_read.argtypes = [P, C.POINTER(C.c_double * 8), C.POINTER(C.c_long * 8)]
+        L.gkm_flux_batch.argtypes = [P, C.c_long, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.gkm_weno_batch.argtypes = [P, C.c_long, _dp, C.c_int, _dp]
+        L.lpt_upload.argtypes = [P, C.c_long, _dp]
+        L.lpt_download.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.lpt_sample.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lpt_stats.argtypes = [P, C.POINTER(PStats)]
+        L.lpt_coupled_step.argtypes = [P, C.c_double]
+        L.lpt_heun.argtypes = [P, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One resident GKM-WENO run on one GPU (wraps gkm_ctx)."""
+
+    def __init__(self, nc, params, cfl=0.5, precision=GKM_FP64, prandtl=0.70, device=0, check_state=1):
+        self.nc = nc
+        self.nt = nc + 6
+        cfg = Config(nc=nc, precision=precision, gam=params["gam"], rho0=params.get("rho0", 1.0),
+                     Mt=params.get("Mt", 0.3), Re=params.get("Re", 30.0), mu0=params["mu0"], p0=params["p0"],
+                     t0=params.get("t0", 1.0), cfl=cfl, prandtl=prandtl, device=device, check_state=check_state)
+        h = C.c_void_p()
+        rc = lib().gkm_create(C.byref(cfg), C.byref(h))
+        self._h = h
+        if rc:
+            msg = lib().gkm_last_error(h).decode() if h.value else "gkm_create failed"
+            if h.value:
+                lib().gkm_destroy(h)
+            self._h = C.c_void_p()
+            raise GkmError(rc, msg)
+
+    def _chk(self, rc):
+        if rc:
+            raise GkmError(rc, lib().gkm_last_error(self._h).decode())
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().gkm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def K(self):
+        return lib().gkm_get_K(self._h)
+
+    def upload(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        if W.shape != (5, self.nt, self.nt, self.nt):
+            raise ValueError(f"state must be (5, {self.nt}, {self.nt}, {self.nt})")
+        self._chk(lib().gkm_upload_state_packed(self._h, W))
+
+    def download(self):
+        W = np.empty((5, self.nt, self.nt, self.nt))
+        self._chk(lib().gkm_download_state_packed(self._h, W))
+        return W
+
+    def step(self, dt):
+        self._chk(lib().gkm_step(self._h, dt))
+
+    def cfl_dt(self):
+        d = C.c_double()
+        self._chk(lib().gkm_cfl_dt(self._h, C.byref(d)))
+        return d.value
+
+    def run(self, nsteps, record_dt=False):
+        hist = np.empty(max(nsteps, 1)) if record_dt else None
+        self._chk(lib().gkm_run(self._h, nsteps, _ptr(hist)))
+        return hist[:nsteps] if record_dt else None
+
+    def sync(self):
+        self._chk(lib().gkm_sync(self._h))
+
+    def stats(self):
+        s = Stats()
+        self._chk(lib().gkm_stats_compute(self._h, C.byref(s)))
+        return s
+
+    def timing(self, on=True):
+        self._chk(lib().gkm_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self):
+        ms = (C.c_double * 8)()
+        cnt = (C.c_long * 8)()
+        self._chk(lib().gkm_timing_read(self._h, C.byref(ms), C.byref(cnt)))
+        return np.array(ms[:]), np.array(cnt[:])
+
+    def flux_batch(self, X, tau, dt, precision=GKM_FP64):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 50)
+        tau = np.ascontiguousarray(tau, dtype=np.float64).reshape(-1)
+        F = np.empty((X.shape[0], 5))
+        self._chk(lib().gkm_flux_batch(self._h, X.shape[0], X, tau, dt, precision, F))
+        return F
+
+    def weno_batch(self, S, precision=GKM_FP64):
+        S = np.ascontiguousarray(S, dtype=np.float64).reshape(-1, 30)
+        out = np.empty((S.shape[0], 10))
+        self._chk(lib().gkm_weno_batch(self._h, S.shape[0], S, precision, out))
+        return out
+
+    # ---- particles
+    def lpt_upload(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.np = pos.shape[0]
+        self._chk(lib().lpt_upload(self._h, self.np, pos))
+
+    def lpt_download(self):
+        pos = np.empty((self.np, 3))
+        vel = np.empty((self.np, 3))
+        self._chk(lib().lpt_download(self._h, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def lpt_sample(self):
+        U, gU, gV, gW = (np.empty((self.np, 3)) for _ in range(4))
+        self._chk(lib().lpt_sample(self._h, _ptr(U), _ptr(gU), _ptr(gV), _ptr(gW)))
+        return U, gU, gV, gW
+
+    def lpt_stats(self):
+        s = PStats()
+        self._chk(lib().lpt_stats(self._h, C.byref(s)))
+        return s
+
+    def lpt_coupled_step(self, dt):
+        self._chk(lib().lpt_coupled_step(self._h, dt))
+
+    def lpt_heun(self, dt):
+        self._chk(lib().lpt_heun(self._h, dt))

@@ -1,0 +1,606 @@
+// gkm_api.hip -- C ABI over the gfx950 kernels (declared in include/dnsgkm.h).
+//
+// A context owns one HIP stream (default priority) and all device memory of a
+// run: the resident state (double-buffered), the flux-difference accumulator,
+// a padded staging buffer for host transfers, the device dt scalar and the
+// reduction scratch.  Every HIP call is checked; failures set the context's
+// last-error string and return a GKM_E* code.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/dnsgkm.h"
+#include "gkm_kernels.hpp"
+#include "lpt_kernels.hpp"
+
+struct gkm_ctx {
+    gkm_config cfg{};
+    int nc = 0;
+    long plane = 0;     // nc^3
+    long padded = 0;    // (nc+6)^3
+    double dx = 0.0;
+    gkm::Gas<double> gas{};
+    hipStream_t stream = nullptr;
+    double* W[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* dW = nullptr;
+    double* stage = nullptr;       // 5 * padded
+    double* dscal = nullptr;       // [0] dt, [1] vmax
+    int* dflag = nullptr;          // non-physical-state flag
+    double* partial = nullptr;     // CFL block maxima
+    bool have_state = false;
+    std::string err;
+    // per-kernel timing
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double tms[8] = {0};
+    long tcount[8] = {0};
+    // particles
+    lpt::State lpt;
+};
+
+namespace {
+
+int fail(gkm_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                            \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail((ctx), GKM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+// run f() bracketed by timing events when enabled; accumulates into slot
+template <typename Fn>
+int timed(gkm_ctx* c, int slot, Fn f)
+{
+    if (!c->timing) {
+        hipError_t e = f();
+        if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return GKM_OK;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    hipError_t e = f();
+    if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->tms[slot] += ms;
+    c->tcount[slot] += 1;
+    return GKM_OK;
+}
+
+int eulerian_step(gkm_ctx* c)
+{
+    gkm::SweepArgs a{};
+    a.W = c->W[c->cur];
+    a.dW = c->dW;
+    a.Wout = c->W[1 - c->cur];
+    a.dt = c->dscal;
+    a.plane = c->plane;
+    a.nc = c->nc;
+    a.dx = c->dx;
+    a.inv_dx = 1.0 / c->dx;
+    a.g = c->gas;
+    for (int dir = 0; dir < 3; dir++) {
+        a.seg = gkm::sweep_seg_len(dir, c->nc);
+        a.nseg = c->nc / a.seg;
+        a.mode = dir;  // x: dW = dF, y: dW += dF, z: Wout = W - (dW + dF)/dx
+        int rc = timed(c, dir, [&] { return gkm::launch_sweep(dir, a, c->stream); });
+        if (rc) return rc;
+    }
+    c->cur = 1 - c->cur;
+    c->lpt.valid = false;  // the resident spline described the previous state
+    return GKM_OK;
+}
+
+int cfl_on_device(gkm_ctx* c)
+{
+    return timed(c, 3, [&] {
+        return gkm::launch_cfl(c->W[c->cur], c->plane, c->gas.gam, c->cfg.cfl, c->dx, c->partial, c->dscal,
+                               c->dscal + 1, c->dflag, c->stream);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int gkm_create(const gkm_config* cfg, gkm_ctx** out)
+{
+    if (!cfg || !out) return GKM_EINVAL;
+    *out = nullptr;
+    gkm_ctx* c = new (std::nothrow) gkm_ctx();
+    if (!c) return GKM_ENOMEM;
+    c->cfg = *cfg;
+    const int nc = cfg->nc;
+    auto bad = [&](const std::string& m) {
+        // keep the message reachable: return the context so last_error works
+        c->err = m;
+        *out = c;
+        return GKM_EINVAL;
+    };
+    if (nc > 1024 || !gkm::sweep_grid_ok(nc)) return bad("nc must be a multiple of 16 in [32, 1024]");
+    if (!(cfg->gam > 1.0) || !(cfg->p0 > 0.0) || !(cfg->rho0 > 0.0) || !(cfg->mu0 >= 0.0))
+        return bad("gam must exceed 1 and p0, rho0 must be positive");
+    if (!(cfg->cfl > 0.0) || !(cfg->prandtl > 0.0)) return bad("cfl and prandtl must be positive");
+    if (cfg->precision != GKM_FP64 && cfg->precision != GKM_MIXED) return bad("precision must be GKM_FP64 or GKM_MIXED");
+    const double pi = std::atan(1.0) * 4.0;
+    c->nc = nc;
+    c->plane = (long)nc * nc * nc;
+    c->padded = (long)(nc + 6) * (nc + 6) * (nc + 6);
+    c->dx = 2.0 * pi / nc;                          // FlowField.cpp:46
+    const int K = int((5.0 - 3.0 * cfg->gam) / (cfg->gam - 1.0));  // FlowField.cpp:42
+    const double R = 287.0;                         // param.h:21
+    c->gas = gkm::Gas<double>{cfg->gam, R, 1.0 / cfg->prandtl, cfg->mu0, cfg->p0 / (R * cfg->rho0), 1e-10, K};
+
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) return bad(std::string("hipSetDevice: ") + hipGetErrorString(e));
+    *out = c;
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreate(&c->ev[0]));
+    HIPCHK(c, hipEventCreate(&c->ev[1]));
+    const size_t sbytes = sizeof(double) * 5 * (size_t)c->plane;
+    for (int b = 0; b < 2; b++)
+        if (hipMalloc(&c->W[b], sbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc state");
+    if (hipMalloc(&c->dW, sbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc dW");
+    if (hipMalloc(&c->stage, sizeof(double) * 5 * (size_t)c->padded) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc staging");
+    if (hipMalloc(&c->dscal, sizeof(double) * 4) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc scalars");
+    if (hipMalloc(&c->dflag, sizeof(int) * 4) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc flag");
+    if (hipMalloc(&c->partial, sizeof(double) * gkm::kCflBlocks) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc partial");
+    return GKM_OK;
+}
+
+void gkm_destroy(gkm_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    lpt::release(c->lpt);
+    hipFree(c->W[0]);
+    hipFree(c->W[1]);
+    hipFree(c->dW);
+    hipFree(c->stage);
+    hipFree(c->dscal);
+    hipFree(c->dflag);
+    hipFree(c->partial);
+    if (c->ev[0]) hipEventDestroy(c->ev[0]);
+    if (c->ev[1]) hipEventDestroy(c->ev[1]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* gkm_last_error(const gkm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gkm_get_K(const gkm_ctx* c) { return c ? c->gas.K : -1; }
+
+int gkm_upload_state(gkm_ctx* c, const double* const W[5])
+{
+    if (!c || !W) return fail(c, GKM_EINVAL, "null argument");
+    for (int q = 0; q < 5; q++) {
+        if (!W[q]) return fail(c, GKM_EINVAL, "null state plane");
+        HIPCHK(c, hipMemcpyAsync(c->stage + q * c->padded, W[q], sizeof(double) * c->padded, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    int rc = timed(c, 7, [&] { return gkm::launch_unpack(c->stage, c->W[c->cur], c->nc, c->stream); });
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return GKM_OK;
+}
+
+int gkm_upload_state_packed(gkm_ctx* c, const double* W5)
+{
+    if (!c || !W5) return fail(c, GKM_EINVAL, "null argument");
+    const double* W[5];
+    for (int q = 0; q < 5; q++) W[q] = W5 + q * c->padded;
+    return gkm_upload_state(c, W);
+}
+
+int gkm_download_state(gkm_ctx* c, double* const W[5])
+{
+    if (!c || !W) return fail(c, GKM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    int rc = timed(c, 7, [&] { return gkm::launch_pack(c->W[c->cur], c->stage, c->nc, c->stream); });
+    if (rc) return rc;
+    for (int q = 0; q < 5; q++) {
+        if (!W[q]) return fail(c, GKM_EINVAL, "null state plane");
+        HIPCHK(c, hipMemcpyAsync(W[q], c->stage + q * c->padded, sizeof(double) * c->padded, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int gkm_download_state_packed(gkm_ctx* c, double* W5)
+{
+    if (!c || !W5) return fail(c, GKM_EINVAL, "null argument");
+    double* W[5];
+    for (int q = 0; q < 5; q++) W[q] = W5 + q * c->padded;
+    return gkm_download_state(c, W);
+}
+
+int gkm_step(gkm_ctx* c, double dt)
+{
+    if (!c) return GKM_EINVAL;
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, GKM_EINVAL, "dt must be positive and finite");
+    HIPCHK(c, hipMemcpyAsync(c->dscal, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // dt lives on the host stack
+    return eulerian_step(c);
+}
+
+int gkm_cfl_dt(gkm_ctx* c, double* dt)
+{
+    if (!c || !dt) return fail(c, GKM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    int rc = cfl_on_device(c);
+    if (rc) return rc;
+    double s[2];
+    int flag = 0;
+    HIPCHK(c, hipMemcpyAsync(s, c->dscal, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&flag, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flag && c->cfg.check_state) return fail(c, GKM_ESTATE, "non-physical state (rho <= 0, p <= 0 or NaN)");
+    *dt = s[0];
+    return GKM_OK;
+}
+
+int gkm_run(gkm_ctx* c, int nsteps, double* dt_hist)
+{
+    if (!c || nsteps < 0) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    for (int n = 0; n < nsteps; n++) {
+        int rc = cfl_on_device(c);  // dt of this step from W^n (update.cpp:115, 167)
+        if (rc) return rc;
+        if (dt_hist) HIPCHK(c, hipMemcpyAsync(dt_hist + n, c->dscal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        rc = eulerian_step(c);
+        if (rc) return rc;
+    }
+    int flag = 0;
+    HIPCHK(c, hipMemcpyAsync(&flag, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flag && c->cfg.check_state) return fail(c, GKM_ESTATE, "non-physical state (rho <= 0, p <= 0 or NaN)");
+    return GKM_OK;
+}
+
+int gkm_sync(gkm_ctx* c)
+{
+    if (!c) return GKM_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int gkm_timing_enable(gkm_ctx* c, int on)
+{
+    if (!c) return GKM_EINVAL;
+    c->timing = on != 0;
+    for (int i = 0; i < 8; i++) {
+        c->tms[i] = 0.0;
+        c->tcount[i] = 0;
+    }
+    return GKM_OK;
+}
+
+int gkm_timing_read(gkm_ctx* c, double ms[8], long counts[8])
+{
+    if (!c || !ms || !counts) return fail(c, GKM_EINVAL, "null argument");
+    for (int i = 0; i < 8; i++) {
+        ms[i] = c->tms[i];
+        counts[i] = c->tcount[i];
+    }
+    return GKM_OK;
+}
+
+int gkm_flux_batch(gkm_ctx* c, long n, const double* in, const double* tau, double dt, int precision, double* F)
+{
+    if (!c || n < 0 || (n > 0 && (!in || !tau || !F))) return fail(c, GKM_EINVAL, "bad argument");
+    if (n == 0) return GKM_OK;
+    double *din = nullptr, *dtau = nullptr, *dF = nullptr;
+    int rc = GKM_OK;
+    if (hipMalloc(&din, sizeof(double) * 50 * n) != hipSuccess || hipMalloc(&dtau, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&dF, sizeof(double) * 5 * n) != hipSuccess) {
+        rc = fail(c, GKM_ENOMEM, "hipMalloc flux batch");
+    } else {
+        hipError_t e = hipMemcpyAsync(din, in, sizeof(double) * 50 * n, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dtau, tau, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = gkm::launch_flux_batch(n, din, dtau, dt, c->dx, c->gas, precision, dF, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(F, dF, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, GKM_EHIP, std::string("flux batch: ") + hipGetErrorString(e));
+    }
+    hipFree(din);
+    hipFree(dtau);
+    hipFree(dF);
+    return rc;
+}
+
+int gkm_weno_batch(gkm_ctx* c, long n, const double* in, int precision, double* out)
+{
+    if (!c || n < 0 || (n > 0 && (!in || !out))) return fail(c, GKM_EINVAL, "bad argument");
+    if (n == 0) return GKM_OK;
+    double *din = nullptr, *dout = nullptr;
+    int rc = GKM_OK;
+    if (hipMalloc(&din, sizeof(double) * 30 * n) != hipSuccess || hipMalloc(&dout, sizeof(double) * 10 * n) != hipSuccess) {
+        rc = fail(c, GKM_ENOMEM, "hipMalloc weno batch");
+    } else {
+        hipError_t e = hipMemcpyAsync(din, in, sizeof(double) * 30 * n, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = gkm::launch_weno_batch(n, din, c->gas, precision, dout, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, GKM_EHIP, std::string("weno batch: ") + hipGetErrorString(e));
+    }
+    hipFree(din);
+    hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Splines, statistics and particles
+// ===========================================================================
+namespace {
+
+lpt::Geom geom_of(const gkm_ctx* c)
+{
+    lpt::Geom g;
+    g.nc = c->nc;
+    g.nt = c->nc + 6;
+    g.N = c->nc + 8;
+    // Interpolation.cpp:7-9 with sta = 0: start = -3 dx + dx/2, end = start + (nt-1) dx
+    const double s = (0 - 3) * c->dx + 0.5 * c->dx;
+    const double e = s + (g.nt - 1) * c->dx;
+    g.start = s;
+    g.dinv = 1.0 / ((e - s) / (double)(g.nt - 1));
+    return g;
+}
+
+int ensure_spline_mem(gkm_ctx* c)
+{
+    lpt::State& L = c->lpt;
+    if (L.pivots) return GKM_OK;
+    const long N = c->nc + 8;
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < 3; q++)
+            if (hipMalloc(&L.coef[b][q], sizeof(double) * N * N * N) != hipSuccess)
+                return fail(c, GKM_ENOMEM, "hipMalloc spline coefficients");
+    const int n = c->nc + 6 - 2;
+    double* piv = new double[n];
+    for (int r = 0; r < n; r++) piv[r] = 1.0 / (4.0 - (r > 0 ? piv[r - 1] : 0.0));
+    if (hipMalloc(&L.pivots, sizeof(double) * n) != hipSuccess) {
+        delete[] piv;
+        return fail(c, GKM_ENOMEM, "hipMalloc pivots");
+    }
+    hipError_t e = hipMemcpy(L.pivots, piv, sizeof(double) * n, hipMemcpyHostToDevice);
+    delete[] piv;
+    if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("hipMemcpy pivots: ") + hipGetErrorString(e));
+    if (hipMalloc(&L.red, sizeof(double) * 32 * lpt::kStatBlocks) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc reduction scratch");
+    return GKM_OK;
+}
+
+// velocity spline of the resident state into coefficient buffer b
+int build_spline(gkm_ctx* c, int b)
+{
+    int rc = ensure_spline_mem(c);
+    if (rc) return rc;
+    const lpt::Geom g = geom_of(c);
+    return timed(c, 4, [&] {
+        for (int q = 1; q <= 3; q++) {
+            hipError_t e = lpt::launch_prefilter(c->W[c->cur], c->plane, q, g, c->lpt.pivots, c->lpt.coef[b][q - 1],
+                                                 c->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    });
+}
+
+int ensure_spline(gkm_ctx* c)
+{
+    if (c->lpt.valid) return GKM_OK;
+    int rc = build_spline(c, c->lpt.cur);
+    if (rc) return rc;
+    c->lpt.valid = true;
+    return GKM_OK;
+}
+
+int sum_partials(gkm_ctx* c, int stride, int off, int K, double* out)
+{
+    std::vector<double> h((size_t)stride * lpt::kStatBlocks);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->lpt.red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (int b = 0; b < lpt::kStatBlocks; b++) s += h[(size_t)b * stride + off + k];
+        out[k] = s;
+    }
+    return GKM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gkm_stats_compute(gkm_ctx* c, gkm_stats* out)
+{
+    if (!c || !out) return fail(c, GKM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    int rc = ensure_spline(c);
+    if (rc) return rc;
+    const lpt::Geom g = geom_of(c);
+    const lpt::StatParams sp{c->gas.gam, c->gas.R, c->gas.mu0, c->gas.T0};
+    const double* cu[3] = {c->lpt.coef[c->lpt.cur][0], c->lpt.coef[c->lpt.cur][1], c->lpt.coef[c->lpt.cur][2]};
+    hipError_t e = lpt::launch_stats(c->W[c->cur], c->plane, cu, g, sp, c->lpt.red, c->stream);
+    if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("stats: ") + hipGetErrorString(e));
+    double s[16];
+    rc = sum_partials(c, 32, 0, 16, s);
+    if (rc) return rc;
+    const double Nc = (double)c->plane;
+    out->tke = s[0] / Nc;
+    out->diss = s[1] / Nc;
+    out->sijsij = s[2] / Nc;
+    out->enstrophy = s[3] / Nc;
+    for (int a = 0; a < 3; a++) {
+        const double m2 = s[4 + a] / Nc, m3 = s[7 + a] / Nc, m4 = s[10 + a] / Nc;
+        out->skew[a] = m3 / std::pow(m2, 1.5);
+        out->kurt[a] = m4 / (m2 * m2);
+    }
+    const double pm = s[13] / Nc, tm = s[14] / Nc, vm = s[15] / Nc;
+    e = lpt::launch_stats2(c->W[c->cur], c->plane, sp, pm, tm, vm, c->lpt.red, c->stream);
+    if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("stats2: ") + hipGetErrorString(e));
+    double r[3];
+    rc = sum_partials(c, 32, 16, 3, r);
+    if (rc) return rc;
+    const double Mt = c->cfg.Mt, gam = c->gas.gam;
+    out->pmean = pm;
+    out->tmean = tm;
+    out->vmean = vm;
+    out->prms = std::sqrt(r[0] / Nc) / (gam * c->cfg.p0 * Mt * Mt);         // update.cpp:825-833
+    out->trms = std::sqrt(r[1] / Nc) / ((gam - 1) * c->gas.T0 * Mt * Mt);
+    out->vrms = std::sqrt(r[2] / Nc) / (Mt * Mt / c->cfg.rho0);
+    return GKM_OK;
+}
+
+int lpt_upload(gkm_ctx* c, long np, const double* pos)
+{
+    if (!c || np < 0 || (np > 0 && !pos)) return fail(c, GKM_EINVAL, "bad argument");
+    lpt::State& L = c->lpt;
+    hipFree(L.pos);
+    hipFree(L.U);
+    hipFree(L.G);
+    L.pos = L.U = L.G = nullptr;
+    L.np = np;
+    L.sampled = false;
+    if (np == 0) return GKM_OK;
+    if (hipMalloc(&L.pos, sizeof(double) * 3 * np) != hipSuccess || hipMalloc(&L.U, sizeof(double) * 3 * np) != hipSuccess ||
+        hipMalloc(&L.G, sizeof(double) * 9 * np) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc particles");
+    const double twopi = 2.0 * (std::atan(1.0) * 4.0);
+    for (long p = 0; p < 3 * np; p++)
+        if (!(pos[p] >= 0.0 && pos[p] <= twopi)) return fail(c, GKM_EINVAL, "particle position outside [0, 2 pi]");
+    HIPCHK(c, hipMemcpyAsync(L.pos, pos, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int lpt_download(gkm_ctx* c, double* pos, double* vel)
+{
+    if (!c) return GKM_EINVAL;
+    const lpt::State& L = c->lpt;
+    if (L.np == 0) return GKM_OK;
+    if (pos) HIPCHK(c, hipMemcpyAsync(pos, L.pos, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
+    if (vel) HIPCHK(c, hipMemcpyAsync(vel, L.U, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+static int sample_device(gkm_ctx* c)
+{
+    int rc = ensure_spline(c);
+    if (rc) return rc;
+    lpt::State& L = c->lpt;
+    const lpt::Geom g = geom_of(c);
+    const double* cu[3] = {L.coef[L.cur][0], L.coef[L.cur][1], L.coef[L.cur][2]};
+    rc = timed(c, 5, [&] { return lpt::launch_sample(L.np, L.pos, cu, g, L.U, L.G, c->stream); });
+    if (rc) return rc;
+    L.sampled = true;
+    return GKM_OK;
+}
+
+int lpt_sample(gkm_ctx* c, double* U, double* gU, double* gV, double* gW)
+{
+    if (!c) return GKM_EINVAL;
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    lpt::State& L = c->lpt;
+    if (L.np == 0) return GKM_OK;
+    int rc = sample_device(c);
+    if (rc) return rc;
+    if (U) HIPCHK(c, hipMemcpyAsync(U, L.U, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
+    if (gU || gV || gW) {
+        std::vector<double> G((size_t)9 * L.np);
+        HIPCHK(c, hipMemcpyAsync(G.data(), L.G, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double* outs[3] = {gU, gV, gW};
+        for (int f = 0; f < 3; f++)
+            if (outs[f])
+                for (long p = 0; p < L.np; p++)
+                    for (int a = 0; a < 3; a++) outs[f][3 * p + a] = G[(size_t)9 * p + 3 * f + a];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int lpt_stats(gkm_ctx* c, gkm_pstats* out)
+{
+    if (!c || !out) return fail(c, GKM_EINVAL, "null argument");
+    lpt::State& L = c->lpt;
+    if (!L.sampled) return fail(c, GKM_ENOTREADY, "lpt_stats needs a sample");
+    int rc = ensure_spline_mem(c);
+    if (rc) return rc;
+    hipError_t e = lpt::launch_pstats(L.np, L.U, L.G, L.red, c->stream);
+    if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("pstats: ") + hipGetErrorString(e));
+    double s[6];
+    rc = sum_partials(c, 8, 0, 6, s);
+    if (rc) return rc;
+    const double n = (double)L.np;  // update.cpp:1147-1152 divide by numparticles
+    out->tke = s[0] / n;
+    out->sijsij = s[1] / n;
+    out->skew_u = (s[3] / n) / std::pow(s[2] / n, 1.5);
+    out->skew_w = (s[5] / n) / std::pow(s[4] / n, 1.5);
+    return GKM_OK;
+}
+
+int lpt_heun(gkm_ctx* c, double dt)
+{
+    if (!c) return GKM_EINVAL;
+    lpt::State& L = c->lpt;
+    if (!L.sampled) return fail(c, GKM_ENOTREADY, "lpt_heun needs the start-of-step sample");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, GKM_EINVAL, "dt must be positive and finite");
+    int rc = ensure_spline(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->dscal + 2, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const lpt::Geom g = geom_of(c);
+    const double* cn[3] = {L.coef[L.cur][0], L.coef[L.cur][1], L.coef[L.cur][2]};
+    rc = timed(c, 5, [&] { return lpt::launch_heun(L.np, L.pos, L.U, cn, g, c->dscal + 2, c->stream); });
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int lpt_coupled_step(gkm_ctx* c, double dt)
+{
+    if (!c) return GKM_EINVAL;
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, GKM_EINVAL, "dt must be positive and finite");
+    lpt::State& L = c->lpt;
+    int rc = sample_device(c);  // U^n at x^n from the spline of W^n (update.cpp:136-137)
+    if (rc) return rc;
+    rc = gkm_step(c, dt);       // W^{n+1} (update.cpp:145-156)
+    if (rc) return rc;
+    const int nb = 1 - L.cur;   // spline of W^{n+1} (update.cpp:157)
+    rc = build_spline(c, nb);
+    if (rc) return rc;
+    L.cur = nb;
+    L.valid = true;
+    const lpt::Geom g = geom_of(c);
+    const double* cn[3] = {L.coef[nb][0], L.coef[nb][1], L.coef[nb][2]};
+    rc = timed(c, 5, [&] { return lpt::launch_heun(L.np, L.pos, L.U, cn, g, c->dscal, c->stream); });
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// gkm_kernels.hpp -- launch interface of the gfx950 kernels of the Eulerian step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "gkm_math.hpp"
+
+namespace gkm {
+
+// Device state: 5 planes (rho, rho u, rho v, rho w, rho E) of nc^3 doubles,
+// unpadded and periodic, cell (y,x,z) at (y*nc + x)*nc + z.  Ghost layers exist
+// only in the host layout; the kernels wrap indices instead.
+struct SweepArgs {
+    const double* W;     // state at t^n
+    double* dW;          // sum of flux differences of the sweeps done so far
+    double* Wout;        // state at t^{n+1} (written by the last sweep)
+    const double* dt;    // device scalar (graph-capturable)
+    long plane;          // nc^3
+    int nc;
+    int seg;             // faces per workgroup along the sweep axis
+    int nseg;            // nc / seg
+    int mode;            // 0: dW = dF   1: dW += dF   2: Wout = W - (dW + dF)/dx
+    double dx, inv_dx;
+    Gas<double> g;
+};
+
+// Sweep tile shapes (faces along the sweep axis x two tangential axes).
+template <int DIR> struct Tile;
+template <> struct Tile<0> { static constexpr int CN = 4, CA = 4, CB = 16; };   // x sweep: a = y, b = z
+template <> struct Tile<1> { static constexpr int CN = 4, CA = 4, CB = 16; };   // y sweep: a = x, b = z
+template <> struct Tile<2> { static constexpr int CN = 16, CA = 4, CB = 4; };   // z sweep: a = x, b = y
+
+hipError_t launch_sweep(int dir, const SweepArgs& a, hipStream_t s);
+int sweep_seg_len(int dir, int nc);  // faces per workgroup chosen for a grid
+bool sweep_grid_ok(int nc);          // nc compatible with every tile shape
+
+// CFL: partial maxima (one per block) then a one-block final pass writing
+// dt = cfl*dx/max and a non-physical-state flag.
+constexpr int kCflBlocks = 1024;
+hipError_t launch_cfl(const double* W, long plane, double gam, double cfl, double dx, double* partial, double* dt,
+                      double* vmax, int* bad, hipStream_t s);
+
+// padded (nt^3 per plane, ghosts) <-> unpadded (nc^3 per plane)
+hipError_t launch_unpack(const double* padded, double* W, int nc, hipStream_t s);
+hipError_t launch_pack(const double* W, double* padded, int nc, hipStream_t s);
+
+hipError_t launch_flux_batch(long n, const double* in, const double* tau, double dt, double dx, const Gas<double>& g,
+                             int precision, double* F, hipStream_t s);
+hipError_t launch_weno_batch(long n, const double* in, const Gas<double>& g, int precision, double* out,
+                             hipStream_t s);
+
+}  // namespace gkm

@@ -1,0 +1,394 @@
+// gkm_math.hpp -- pointwise math of the GKM-WENO step for gfx950 kernels.
+//
+// Everything here is __host__ __device__ and templated on the arithmetic type
+// T (double for the FP64 path, float for the mixed-precision path), so the same
+// source runs in the sweep kernels and in host-side unit checks.
+//
+// Reference semantics followed (nishantp9/DNS_GKM_LPT, src/):
+//   c2p                GKMflux.cpp:302-309
+//   WENO-SYMOO         update.cpp:556-680 (4 candidate stencils, linear weights
+//                      1/20, 9/20, 9/20, 1/20, IS3 := max IS, alpha = d/(eps+IS),
+//                      energy reconstructed through temperature)
+//   collision time     update.cpp:276-278 (Sutherland, left cell of the face)
+//   GKS flux           GKMflux.cpp:11-229 with MCal GKMflux.cpp:238-278 and
+//                      slopesolver GKMflux.cpp:283-297 (a[4] scaled by 1/Pr)
+// The flux is reorganised by Maxwellian state: the interface terms that belong
+// to the left state, the right state and the equilibrium state are each summed
+// in one block while that state's moment tables are live, and the collision
+// coefficients (which need only the densities and lambdas) are computed first.
+// This changes the order of the floating-point sums, not the terms.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace gkm {
+
+template <typename T>
+struct Gas {
+    T gam;      // ratio of specific heats
+    T R;        // gas constant (param.h:21)
+    T inv_prn;  // 1/Pr (param.h:19)
+    T mu0;      // Sutherland reference viscosity (FlowField.cpp:50)
+    T T0;       // reference temperature p0/(R rho0) (FlowField.cpp:47)
+    T eps;      // WENO epsilon (param.h:18)
+    int K;      // internal degrees of freedom (FlowField.cpp:42)
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ T sq(T a) { return a * a; }
+
+template <typename T> __host__ __device__ __forceinline__ T g_sqrt(T x) { return sqrt(x); }
+template <typename T> __host__ __device__ __forceinline__ T g_exp(T x) { return exp(x); }
+template <typename T> __host__ __device__ __forceinline__ T g_erfc(T x) { return erfc(x); }
+template <> __host__ __device__ __forceinline__ float g_sqrt<float>(float x) { return sqrtf(x); }
+template <> __host__ __device__ __forceinline__ float g_exp<float>(float x) { return expf(x); }
+template <> __host__ __device__ __forceinline__ float g_erfc<float>(float x) { return erfcf(x); }
+
+// ---------------------------------------------------------------------------
+// WENO-SYMOO: reconstruction at face i+1/2 seen from the left, from cells
+// a..f = i-2..i+3 (update.cpp:561-621).  The right state is the same formula
+// on the mirrored stencil (f..a).
+// ---------------------------------------------------------------------------
+template <typename T>
+__host__ __device__ __forceinline__ T weno_side(T a, T b, T c, T d, T e, T f, T eps)
+{
+    const T r3 = T(1) / T(3), r6 = T(1) / T(6), r12 = T(1) / T(12);
+    T q0 = a * r3 - T(7) * b * r6 + T(11) * c * r6;
+    T q1 = -b * r6 + T(5) * c * r6 + d * r3;
+    T q2 = c * r3 + T(5) * d * r6 - e * r6;
+    T q3 = T(11) * d * r6 - T(7) * e * r6 + f * r3;
+    T s0 = T(13) * sq(a - T(2) * b + c) * r12 + sq(a - T(4) * b + T(3) * c) * T(0.25);
+    T s1 = T(13) * sq(b - T(2) * c + d) * r12 + sq(-b + d) * T(0.25);
+    T s2 = T(13) * sq(c - T(2) * d + e) * r12 + sq(-T(3) * c + T(4) * d - e) * T(0.25);
+    T s3 = T(13) * sq(d - T(2) * e + f) * r12 + sq(-T(5) * d + T(8) * e - T(3) * f) * T(0.25);
+    T smax = s0 > s1 ? s0 : s1;
+    smax = smax > s2 ? smax : s2;
+    s3 = smax > s3 ? smax : s3;  // update.cpp:577-584
+    T a0 = T(0.05) / (eps + s0);
+    T a1 = T(0.45) / (eps + s1);
+    T a2 = T(0.45) / (eps + s2);
+    T a3 = T(0.05) / (eps + s3);
+    T inv = T(1) / (a0 + a1 + a2 + a3);
+    return (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) * inv;
+}
+
+// temperature of a conservative state (update.cpp:624-629)
+template <typename T>
+__host__ __device__ __forceinline__ T temp_of(const T W[5], T gam, T R)
+{
+    T ir = T(1) / W[0];
+    return (gam - T(1)) * (W[4] * ir - T(0.5) * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) * ir * ir) / R;
+}
+
+// ApplyWENO (update.cpp:556-680): S[s][q] for cells i-2..i+3, out Wl, Wr.
+template <typename T>
+__host__ __device__ __forceinline__ void weno5(const T S[6][5], const Gas<T>& g, T Wl[5], T Wr[5])
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        Wl[q] = weno_side(S[0][q], S[1][q], S[2][q], S[3][q], S[4][q], S[5][q], g.eps);
+        Wr[q] = weno_side(S[5][q], S[4][q], S[3][q], S[2][q], S[1][q], S[0][q], g.eps);
+    }
+    T Tc[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) Tc[s] = temp_of(S[s], g.gam, g.R);
+    T Tl = weno_side(Tc[0], Tc[1], Tc[2], Tc[3], Tc[4], Tc[5], g.eps);
+    T Tr = weno_side(Tc[5], Tc[4], Tc[3], Tc[2], Tc[1], Tc[0], g.eps);
+    T igm = T(1) / (g.gam - T(1));
+    Wl[4] = Wl[0] * g.R * Tl * igm + T(0.5) * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
+    Wr[4] = Wr[0] * g.R * Tr * igm + T(0.5) * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+}
+
+// collision time mu(T)/p of the face's left cell, Sutherland law (update.cpp:276-278)
+template <typename T>
+__host__ __device__ __forceinline__ T collision_time(const T W[5], const Gas<T>& g)
+{
+    T ir = T(1) / W[0];
+    T u = W[1] * ir, v = W[2] * ir, w = W[3] * ir;
+    T P = (W[0] * (g.gam - T(1))) * (W[4] * ir - T(0.5) * (u * u + v * v + w * w));
+    T Tt = P / (g.R * W[0]);
+    T x = Tt / g.T0;
+    T mu = g.mu0 * (x * g_sqrt(x)) * (g.T0 + T(110.4)) / (Tt + T(110.4));
+    return mu / P;
+}
+
+// ---------------------------------------------------------------------------
+// Maxwellian moments.  For a state with mean (U,V,W) and lambda, the full
+// moments <u^n> satisfy f_n = U f_{n-1} + (n-1)/(2 lambda) f_{n-2}, f_0 = 1,
+// f_1 = U; the half-space moments over u>0 (u<0) start from
+// 0.5 erfc(-+sqrt(lambda) U) and U h_0 +- exp(-lambda U^2)/(2 sqrt(pi lambda))
+// and follow the same recursion (GKMflux.cpp:50-68).  Internal dof:
+// <xi^2> = K/(2 lambda), <xi^4> = K(K+2)/(4 lambda^2) (GKMflux.cpp:45-48).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Maxw {
+    T den, u, v, w, lam, i2l, e2, e4;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ void maxw_init(Maxw<T>& s, const T W[5], const Gas<T>& g)
+{
+    s.den = W[0];
+    T ir = T(1) / W[0];
+    s.u = W[1] * ir;
+    s.v = W[2] * ir;
+    s.w = W[3] * ir;
+    T P = (s.den * (g.gam - T(1))) * (W[4] * ir - T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w));
+    s.lam = T(0.5) * s.den / P;
+    s.i2l = P / s.den;  // 1/(2 lambda)
+    s.e2 = T(g.K) * s.i2l;
+    s.e4 = T(g.K * (g.K + 2)) * s.i2l * s.i2l;
+}
+
+template <int N, typename T>
+__host__ __device__ __forceinline__ void full_tab(T m, T i2l, T* f)
+{
+    f[0] = T(1);
+    f[1] = m;
+#pragma unroll
+    for (int n = 2; n < N; n++) f[n] = m * f[n - 1] + T(n - 1) * i2l * f[n - 2];
+}
+
+template <int N, typename T>
+__host__ __device__ __forceinline__ void half_tab(T m, T i2l, T h0, T h1, T* h)
+{
+    h[0] = h0;
+    h[1] = h1;
+#pragma unroll
+    for (int n = 2; n < N; n++) h[n] = m * h[n - 1] + T(n - 1) * i2l * h[n - 2];
+}
+
+// <u^k v^l w^m psi>, psi = (1, u, v, w, (u^2+v^2+w^2+xi^2)/2)  (GKMflux.cpp:269-278)
+template <int k, int l, int m, typename T>
+__host__ __device__ __forceinline__ void mom(const T* tu, const T* tv, const T* tw, T e2, T M[5])
+{
+    T t = tu[k] * tv[l] * tw[m];
+    M[0] = t;
+    M[1] = tu[k + 1] * tv[l] * tw[m];
+    M[2] = tu[k] * tv[l + 1] * tw[m];
+    M[3] = tu[k] * tv[l] * tw[m + 1];
+    M[4] = T(0.5) * (tu[k + 2] * tv[l] * tw[m] + tu[k] * tv[l + 2] * tw[m] + tu[k] * tv[l] * tw[m + 2] + t * e2);
+}
+
+// acc += c * <u^k v^l w^m psi (a . psi)>  (GKMflux.cpp:238-267)
+template <int k, int l, int m, typename T>
+__host__ __device__ __forceinline__ void mom_a(const T* tu, const T* tv, const T* tw, T e2, T e4, const T a[5], T c,
+                                               T acc[5])
+{
+#define T3(p, q, r) (tu[p] * tv[q] * tw[r])
+    T h0 = T(0.5) * (T3(2 + k, l, m) + T3(k, 2 + l, m) + T3(k, l, 2 + m) + T3(k, l, m) * e2);
+    T h1 = T(0.5) * (T3(3 + k, l, m) + T3(1 + k, 2 + l, m) + T3(1 + k, l, 2 + m) + T3(1 + k, l, m) * e2);
+    T h2 = T(0.5) * (T3(2 + k, 1 + l, m) + T3(k, 3 + l, m) + T3(k, 1 + l, 2 + m) + T3(k, 1 + l, m) * e2);
+    T h3 = T(0.5) * (T3(2 + k, l, 1 + m) + T3(k, 2 + l, 1 + m) + T3(k, l, 3 + m) + T3(k, l, 1 + m) * e2);
+    T m0 = a[0] * T3(k, l, m) + a[1] * T3(1 + k, l, m) + a[2] * T3(k, 1 + l, m) + a[3] * T3(k, l, 1 + m) + a[4] * h0;
+    T m1 = a[0] * T3(1 + k, l, m) + a[1] * T3(2 + k, l, m) + a[2] * T3(1 + k, 1 + l, m) + a[3] * T3(1 + k, l, 1 + m) +
+           a[4] * h1;
+    T m2 = a[0] * T3(k, 1 + l, m) + a[1] * T3(1 + k, 1 + l, m) + a[2] * T3(k, 2 + l, m) + a[3] * T3(k, 1 + l, 1 + m) +
+           a[4] * h2;
+    T m3 = a[0] * T3(k, l, 1 + m) + a[1] * T3(1 + k, l, 1 + m) + a[2] * T3(k, 1 + l, 1 + m) + a[3] * T3(k, l, 2 + m) +
+           a[4] * h3;
+    T m4 = T(0.25) * a[4] *
+               (T3(4 + k, l, m) + T3(k, 4 + l, m) + T3(k, l, 4 + m) + T3(k, l, m) * e4 +
+                T(2) * (T3(2 + k, 2 + l, m) + T3(2 + k, l, 2 + m) + T3(k, 2 + l, 2 + m)) +
+                T(2) * e2 * (T3(2 + k, l, m) + T3(k, 2 + l, m) + T3(k, l, 2 + m))) +
+           a[0] * h0 + a[1] * h1 + a[2] * h2 + a[3] * h3;
+#undef T3
+    acc[0] += c * m0;
+    acc[1] += c * m1;
+    acc[2] += c * m2;
+    acc[3] += c * m3;
+    acc[4] += c * m4;
+}
+
+// micro slope a from macro slope b (GKMflux.cpp:283-297), a[4] scaled by 1/Pr
+template <typename T>
+__host__ __device__ __forceinline__ void slope(const T b[5], const Maxw<T>& s, const Gas<T>& g, T a[5])
+{
+    T R2 = b[1] - s.u * b[0];
+    T R3 = b[2] - s.v * b[0];
+    T R4 = b[3] - s.w * b[0];
+    T q = s.u * s.u + s.v * s.v + s.w * s.w + T(g.K + 3) * s.i2l;
+    T R5 = T(2) * b[4] - b[0] * q;
+    T lam2 = T(2) * s.lam;
+    a[4] = g.inv_prn * (R5 - T(2) * (s.u * R2 + s.v * R3 + s.w * R4)) * (lam2 * lam2) / T(g.K + 3);
+    a[3] = lam2 * R4 - s.w * a[4];
+    a[2] = lam2 * R3 - s.v * a[4];
+    a[1] = lam2 * R2 - s.u * a[4];
+    a[0] = b[0] - a[1] * s.u - a[2] * s.v - a[3] * s.w - T(0.5) * a[4] * q;
+}
+
+// ---------------------------------------------------------------------------
+// GKS time-integrated interface flux (GKMflux.cpp:11-229).
+//   WL, WR : cell states either side of the face (rotated frame)
+//   Wl, Wr : WENO states at the face
+//   D[0..5]: DWxl, DWxr, DWyl, DWyr, DWzl, DWzr
+//   tau    : collision time of the left cell (the pressure-jump term is added here)
+// ---------------------------------------------------------------------------
+template <typename T>
+__host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], const T Wl[5], const T Wr[5],
+                                                  const T D[6][5], T tau, T dt, T dx, const Gas<T>& g, T F[5])
+{
+    const T pi = T(3.141592653589793238462643383279502884);
+    Maxw<T> L, R;
+    maxw_init(L, Wl, g);
+    maxw_init(R, Wr, g);
+
+    // collision time with the artificial pressure-jump term (GKMflux.cpp:156-157)
+    {
+        T pl = L.den / L.lam, pr = R.den / R.lam;
+        tau = tau + dt * (fabs(pl - pr) / fabs(pl + pr));
+    }
+    // time-integration coefficients (GKMflux.cpp:162-167, 197-204)
+    const T ex = g_exp(-dt / tau);
+    const T om = T(1) - ex;
+    const T igm0 = T(1) / (dt - tau * om);
+    const T gm1 = -om * igm0;
+    const T gm2 = (-dt + T(2) * tau * om - dt * ex) * igm0;
+    const T gm3 = -gm1;
+    const T gm4 = (dt * ex - tau * om) * igm0;
+    const T gm5 = tau * gm3;
+    const T Pc = tau * om;
+    const T Qc = -tau * dt * ex + tau * tau * om;
+    const T p0 = dt - Pc;
+    const T p1 = T(0.5) * dt * dt - tau * p0;
+    const T p2 = -tau * p0 + Qc;
+    const T p3 = Pc;
+    const T p4 = -Qc - tau * Pc;
+    const T p5 = -tau * Pc;
+
+    // error-function / Gaussian terms of the half-space tables
+    const T gl = T(0.5) * g_exp(-L.lam * L.u * L.u) / g_sqrt(L.lam * pi);
+    const T hl0 = T(0.5) * g_erfc(-g_sqrt(L.lam) * L.u);
+    const T gr = T(0.5) * g_exp(-R.lam * R.u * R.u) / g_sqrt(R.lam * pi);
+    const T hr0 = T(0.5) * g_erfc(g_sqrt(R.lam) * R.u);
+
+    // equilibrium state at the face (GKMflux.cpp:73-80)
+    T We[5];
+    {
+        T hu[3], tv[3], tw[3], M[5];
+        half_tab<3>(L.u, L.i2l, hl0, L.u * hl0 + gl, hu);
+        full_tab<3>(L.v, L.i2l, tv);
+        full_tab<3>(L.w, L.i2l, tw);
+        mom<0, 0, 0>(hu, tv, tw, L.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) We[q] = L.den * M[q];
+        half_tab<3>(R.u, R.i2l, hr0, R.u * hr0 - gr, hu);
+        full_tab<3>(R.v, R.i2l, tv);
+        full_tab<3>(R.w, R.i2l, tw);
+        mom<0, 0, 0>(hu, tv, tw, R.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) We[q] += R.den * M[q];
+    }
+    Maxw<T> E;
+    maxw_init(E, We, g);
+
+    // Per-state sums.  Be accumulates den_e * B_ (GKMflux.cpp:183-190),
+    // F accumulates the flux (GKMflux.cpp:223-228); bye/bze are the
+    // phase-II right-hand sides of the equilibrium slopes (GKMflux.cpp:141-144).
+    T Be[5] = {0, 0, 0, 0, 0}, bye[5] = {0, 0, 0, 0, 0}, bze[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[q] = T(0);
+
+    // ---- left (u>0 half of the left Maxwellian) and right (u<0 half of the right)
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const Maxw<T>& S = side == 0 ? L : R;
+        const int dxi = side == 0 ? 0 : 1;  // index of DWx{l,r} etc. in D
+        T hu[7], fu[6], tv[6], tw[6];
+        if (side == 0)
+            half_tab<7>(S.u, S.i2l, hl0, S.u * hl0 + gl, hu);
+        else
+            half_tab<7>(S.u, S.i2l, hr0, S.u * hr0 - gr, hu);
+        full_tab<6>(S.u, S.i2l, fu);
+        full_tab<6>(S.v, S.i2l, tv);
+        full_tab<6>(S.w, S.i2l, tw);
+        const T iden = T(1) / S.den;
+        T b[5], ax[5], ay[5], az[5], A[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = D[0 + dxi][q] * iden;
+        slope(b, S, g, ax);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = D[2 + dxi][q] * iden;
+        slope(b, S, g, ay);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = D[4 + dxi][q] * iden;
+        slope(b, S, g, az);
+        // phase II (GKMflux.cpp:130-151)
+        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, ay, T(1), bye);
+        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, az, T(1), bze);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = T(0);
+        mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, T(-1), b);
+        mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, T(-1), b);
+        mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, T(-1), b);
+        slope(b, S, g, A);
+        // this state's share of den_e * B_ and of F
+        const T cB = (gm4 + gm5) * S.den;
+        T M[5];
+        mom<0, 0, 0>(hu, tv, tw, S.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) Be[q] += gm3 * S.den * M[q];
+        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cB, Be);
+        mom_a<0, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cB, Be);
+        mom_a<0, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cB, Be);
+        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, A, gm5 * S.den, Be);
+        mom<1, 0, 0>(hu, tv, tw, S.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) F[q] += p3 * S.den * M[q];
+        const T cF = p4 * S.den;
+        mom_a<2, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cF, F);
+        mom_a<1, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cF, F);
+        mom_a<1, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cF, F);
+        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, A, p5 * S.den, F);
+    }
+
+    // ---- equilibrium state
+    {
+        T hp[7], hn[7], fu[7], tv[6], tw[6];
+        const T ge = T(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
+        const T sl = g_sqrt(E.lam) * E.u;
+        const T hp0 = T(0.5) * g_erfc(-sl), hn0 = T(0.5) * g_erfc(sl);
+        half_tab<7>(E.u, E.i2l, hp0, E.u * hp0 + ge, hp);
+        half_tab<7>(E.u, E.i2l, hn0, E.u * hn0 - ge, hn);
+        full_tab<7>(E.u, E.i2l, fu);
+        full_tab<6>(E.v, E.i2l, tv);
+        full_tab<6>(E.w, E.i2l, tw);
+        // slopes phase I / II of the equilibrium state (GKMflux.cpp:115-125, 148-149)
+        const T c = T(2) / (E.den * dx);
+        T b[5], axl[5], axr[5], ay[5], az[5], A[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * c;
+        slope(b, E, g, axl);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * c;
+        slope(b, E, g, axr);
+        slope(bye, E, g, ay);
+        slope(bze, E, g, az);
+        // B_ and the time slope A_ (GKMflux.cpp:183-191)
+        const T cB = gm2 * E.den;
+        T M[5];
+        mom<0, 0, 0>(fu, tv, tw, E.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) Be[q] += gm1 * E.den * M[q];
+        mom_a<1, 0, 0>(hp, tv, tw, E.e2, E.e4, axl, cB, Be);
+        mom_a<1, 0, 0>(hn, tv, tw, E.e2, E.e4, axr, cB, Be);
+        mom_a<0, 1, 0>(fu, tv, tw, E.e2, E.e4, ay, cB, Be);
+        mom_a<0, 0, 1>(fu, tv, tw, E.e2, E.e4, az, cB, Be);
+        const T iden = T(1) / E.den;
+#pragma unroll
+        for (int q = 0; q < 5; q++) Be[q] *= iden;
+        slope(Be, E, g, A);
+        // equilibrium part of the flux (GKMflux.cpp:225)
+        mom<1, 0, 0>(fu, tv, tw, E.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) F[q] += p0 * E.den * M[q];
+        mom_a<1, 0, 0>(fu, tv, tw, E.e2, E.e4, A, p1 * E.den, F);
+        const T cF = p2 * E.den;
+        mom_a<2, 0, 0>(hp, tv, tw, E.e2, E.e4, axl, cF, F);
+        mom_a<2, 0, 0>(hn, tv, tw, E.e2, E.e4, axr, cF, F);
+        mom_a<1, 1, 0>(fu, tv, tw, E.e2, E.e4, ay, cF, F);
+        mom_a<1, 0, 1>(fu, tv, tw, E.e2, E.e4, az, cF, F);
+    }
+}
+
+}  // namespace gkm

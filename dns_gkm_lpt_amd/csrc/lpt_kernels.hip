@@ -1,0 +1,407 @@
+// lpt_kernels.hip -- natural tricubic B-spline prefilter, particle sampling and
+// Heun advection, and flow / particle statistics on gfx950.
+//
+// Spline semantics (einspline UBspline_3d_d with NATURAL ends as built by
+// get_bsplinep, Interpolation.cpp:3-31): nt = nc+6 knots per axis at the
+// padded cell centres, N = nt+2 coefficients per axis, interpolation rows
+// (c_i + 4 c_{i+1} + c_{i+2})/6 = f_i and f'' = 0 at both ends, i.e.
+// c_1 = f_0, c_nt = f_{nt-1}, a [1 4 1] system for the inner coefficients and
+// c_0 = 2c_1 - c_2, c_{nt+1} = 2c_nt - c_{nt-1}.  The 3-D coefficients are the
+// tensor product: the line solve is applied along z, then x, then y.
+#include "lpt_kernels.hpp"
+
+namespace lpt {
+
+void release(State& s)
+{
+    (void)hipFree(s.pos);
+    (void)hipFree(s.U);
+    (void)hipFree(s.G);
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < 3; q++) {
+            (void)hipFree(s.coef[b][q]);
+            s.coef[b][q] = nullptr;
+        }
+    (void)hipFree(s.pivots);
+    (void)hipFree(s.red);
+    s = State();
+}
+
+__device__ __forceinline__ int wrapi(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
+// In-place natural-spline line solve.  The line holds f_0..f_{M-1} at
+// positions 0..M-1 (stride st) and receives c_0..c_{M+1}; Src reads f_p.
+template <typename Src>
+__device__ __forceinline__ void line_solve(int M, const Src& src, double* out, long st, const double* __restrict__ piv)
+{
+    const int n = M - 2;
+    const double c1 = src(0), cM = src(M - 1);
+    double next = src(1);
+    double dp = 0.0;
+    for (int r = 0; r < n; r++) {
+        const double f = next;
+        if (r + 1 < n) next = src(r + 2);  // read before position r+2 is overwritten
+        double rhs = 6.0 * f;
+        if (r == 0) rhs -= c1;
+        if (r == n - 1) rhs -= cM;
+        dp = (rhs - dp) * piv[r];
+        out[(long)(r + 2) * st] = dp;
+    }
+    double x = dp;  // x_{n-1}
+    const double xlast = x;
+    for (int r = n - 2; r >= 0; r--) {
+        x = out[(long)(r + 2) * st] - piv[r] * x;
+        out[(long)(r + 2) * st] = x;
+    }
+    out[st] = c1;
+    out[(long)M * st] = cM;
+    out[0] = 2.0 * c1 - x;                        // x = c_2
+    out[(long)(M + 1) * st] = 2.0 * cM - xlast;   // xlast = c_{M-1}
+}
+
+// pass 1: z lines straight from the state (velocity = W_q / W_0, periodic wrap)
+__global__ void prefilter_z_kernel(const double* __restrict__ W, long plane, int q, Geom g,
+                                   const double* __restrict__ piv, double* __restrict__ coef)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.nt * g.nt) return;
+    const int a = (int)(t / g.nt), b = (int)(t % g.nt);  // padded y, x
+    const int nc = g.nc;
+    const long base = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc;
+    const double* Wq = W + q * plane;
+    auto src = [&](int k) {
+        const long c = base + wrapi(k - 3, nc);
+        return Wq[c] / W[c];
+    };
+    line_solve(g.nt, src, coef + ((long)a * g.N + b) * g.N, 1, piv);
+}
+
+// pass 2 / 3: in place along x (stride N) or y (stride N^2)
+__global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis, const double* __restrict__ piv)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long base, st;
+    if (axis == 1) {  // x lines: a in [0, nt), c in [0, N)
+        if (t >= (long)g.nt * g.N) return;
+        const int a = (int)(t / g.N), c = (int)(t % g.N);
+        base = (long)a * g.N * g.N + c;
+        st = g.N;
+    } else {          // y lines: b, c in [0, N)
+        if (t >= (long)g.N * g.N) return;
+        base = t;
+        st = (long)g.N * g.N;
+    }
+    double* line = coef + base;
+    auto src = [&](int p) { return line[(long)p * st]; };
+    line_solve(g.nt, src, line, st, piv);
+}
+
+static unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_prefilter(const double* W, long plane, int q, const Geom& g, const double* pivots, double* coef,
+                            hipStream_t s)
+{
+    if (q < 1 || q > 3 || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(prefilter_z_kernel, dim3(blocks_for((long)g.nt * g.nt, 64)), dim3(64), 0, s, W, plane, q, g,
+                       pivots, coef);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(blocks_for((long)g.nt * g.N, 64)), dim3(64), 0, s, coef, g, 1,
+                       pivots);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(blocks_for((long)g.N * g.N, 64)), dim3(64), 0, s, coef, g, 2,
+                       pivots);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation (einspline eval_UBspline_3d_d / _vg semantics): u = (p-start)/delta,
+// i = floor(u), t = u - i, weights of the uniform cubic B-spline.  Indices are
+// clamped to the coefficient box so no input can read outside it; every point
+// of the padded grid range is inside the box unclamped.
+// ---------------------------------------------------------------------------
+struct Loc {
+    int i;
+    double w[4], d[4];
+};
+
+__device__ __forceinline__ Loc locate(double p, const Geom& g, bool deriv)
+{
+    double u = (p - g.start) * g.dinv;
+    double ip = floor(u);
+    double t = u - ip;
+    int i = (int)ip;
+    if (i < 0) { i = 0; t = 0.0; }
+    if (i > g.N - 4) { i = g.N - 4; t = 1.0; }
+    Loc L;
+    L.i = i;
+    const double t2 = t * t, t3 = t2 * t;
+    const double s6 = 1.0 / 6.0;
+    L.w[0] = (-t3 + 3 * t2 - 3 * t + 1) * s6;
+    L.w[1] = (3 * t3 - 6 * t2 + 4) * s6;
+    L.w[2] = (-3 * t3 + 3 * t2 + 3 * t + 1) * s6;
+    L.w[3] = t3 * s6;
+    if (deriv) {
+        L.d[0] = (-0.5 * t2 + t - 0.5) * g.dinv;
+        L.d[1] = (1.5 * t2 - 2 * t) * g.dinv;
+        L.d[2] = (-1.5 * t2 + t + 0.5) * g.dinv;
+        L.d[3] = 0.5 * t2 * g.dinv;
+    }
+    return L;
+}
+
+__global__ __launch_bounds__(256) void sample_kernel(long np, const double* __restrict__ pos, const double* __restrict__ cu,
+                                                     const double* __restrict__ cv, const double* __restrict__ cw, Geom g,
+                                                     double* __restrict__ U, double* __restrict__ G)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const Loc Ly = locate(pos[3 * p + 0], g, true);
+    const Loc Lx = locate(pos[3 * p + 1], g, true);
+    const Loc Lz = locate(pos[3 * p + 2], g, true);
+    const long N = g.N;
+    const double* cs[3] = {cu, cv, cw};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        double v = 0, gy = 0, gx = 0, gz = 0;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                const double* row = cs[f] + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
+                double rv = 0, rd = 0;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    rv += Lz.w[c] * row[c];
+                    rd += Lz.d[c] * row[c];
+                }
+                const double wab = Ly.w[a] * Lx.w[b];
+                v += wab * rv;
+                gz += wab * rd;
+                gy += Ly.d[a] * Lx.w[b] * rv;
+                gx += Ly.w[a] * Lx.d[b] * rv;
+            }
+        if (U) U[3 * p + f] = v;
+        if (G) {
+            G[9 * p + 3 * f + 0] = gy;
+            G[9 * p + 3 * f + 1] = gx;
+            G[9 * p + 3 * f + 2] = gz;
+        }
+    }
+}
+
+hipError_t launch_sample(long np, const double* pos, const double* const cu[3], const Geom& g, double* U, double* G,
+                         hipStream_t s)
+{
+    if (np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, cu[0], cu[1], cu[2], g, U, G);
+    return hipGetLastError();
+}
+
+// Heun RK2 of Particle::updatePosition (ParticleTrack.cpp:48-73): predictor
+// x + U dt (not wrapped) sampled on the new field, corrector, wrap to [0, 2pi].
+__global__ __launch_bounds__(256) void heun_kernel(long np, double* __restrict__ pos, const double* __restrict__ U,
+                                                   const double* __restrict__ cu, const double* __restrict__ cv,
+                                                   const double* __restrict__ cw, Geom g, const double* __restrict__ dtp)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double dt = *dtp;
+    const double u = U[3 * p + 0], v = U[3 * p + 1], w = U[3 * p + 2];
+    double y = pos[3 * p + 0], x = pos[3 * p + 1], z = pos[3 * p + 2];
+    const Loc Ly = locate(y + v * dt, g, false);
+    const Loc Lx = locate(x + u * dt, g, false);
+    const Loc Lz = locate(z + w * dt, g, false);
+    const long N = g.N;
+    const double* cs[3] = {cu, cv, cw};
+    double U2[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        double val = 0;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                const double* row = cs[f] + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
+                const double rv = Lz.w[0] * row[0] + Lz.w[1] * row[1] + Lz.w[2] * row[2] + Lz.w[3] * row[3];
+                val += Ly.w[a] * Lx.w[b] * rv;
+            }
+        U2[f] = val;
+    }
+    y += 0.5 * (v + U2[1]) * dt;
+    x += 0.5 * (u + U2[0]) * dt;
+    z += 0.5 * (w + U2[2]) * dt;
+    const double twopi = 2.0 * 3.141592653589793238462643383279502884;
+    double r[3] = {y, x, z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (r[c] > twopi) r[c] = fmod(r[c], twopi);
+        if (r[c] < 0) r[c] = twopi - fmod(-r[c], twopi);
+        pos[3 * p + c] = r[c];
+    }
+}
+
+hipError_t launch_heun(long np, double* pos, const double* U, const double* const cn[3], const Geom& g,
+                       const double* dt, hipStream_t s)
+{
+    if (np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(heun_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, U, cn[0], cn[1], cn[2], g, dt);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Statistics.  Velocity gradients at cell centres come from the spline at its
+// knots, where the uniform cubic B-spline reduces to the 3-point stencils
+// (1,4,1)/6 (value) and (-1,0,1)/(2 delta) (derivative) on the coefficients.
+// Block partial sums (wave shuffles + LDS) go to red[]; the host adds the
+// kStatBlocks partials in a fixed order.
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* out)
+{
+    __shared__ double sm[4][K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        v[k] = x;
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) sm[wv][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < K) out[threadIdx.x] = sm[0][threadIdx.x] + sm[1][threadIdx.x] + sm[2][threadIdx.x] + sm[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void stats_kernel(const double* __restrict__ W, long plane, const double* __restrict__ cu,
+                                                    const double* __restrict__ cv, const double* __restrict__ cw, Geom g,
+                                                    StatParams sp, double* __restrict__ red)
+{
+    // sums: 0 tke, 1 diss, 2 sijsij, 3 enstrophy, 4-6 d2, 7-9 d3, 10-12 d4, 13 P, 14 T, 15 1/rho
+    double acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) acc[k] = 0.0;
+    const int nc = g.nc;
+    const long N = g.N;
+    const double w0 = 1.0 / 6.0, w1 = 4.0 / 6.0;
+    const double d = 0.5 * g.dinv;
+    const double* cs[3] = {cu, cv, cw};
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < plane; c += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(c % nc), i = (int)((c / nc) % nc), j = (int)(c / ((long)nc * nc));
+        const double den = W[c], ir = 1.0 / den;
+        const double uu = W[plane + c] * ir, vv = W[2 * plane + c] * ir, ww = W[3 * plane + c] * ir;
+        const double Vsqr = uu * uu + vv * vv + ww * ww;
+        const double P = (den * (sp.gam - 1)) * (W[4 * plane + c] * ir - 0.5 * Vsqr);
+        const double T = P / (sp.R * den);
+        double val[3], gr[3][3];  // gr[f] = d/dy, d/dx, d/dz
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            // knot (j+3, i+3, k+3): coefficient block starts at (j+3, i+3, k+3) with t = 0
+            const double* base = cs[f] + ((long)(j + 3) * N + (i + 3)) * N + (k + 3);
+            double v = 0, gy = 0, gx = 0, gz = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const double wa = a == 1 ? w1 : w0;
+                const double da = a == 0 ? -d : (a == 2 ? d : 0.0);
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    const double wb = b == 1 ? w1 : w0;
+                    const double db = b == 0 ? -d : (b == 2 ? d : 0.0);
+                    const double* row = base + (a * N + b) * N;
+                    const double rv = w0 * row[0] + w1 * row[1] + w0 * row[2];
+                    const double rd = d * (row[2] - row[0]);
+                    v += wa * wb * rv;
+                    gz += wa * wb * rd;
+                    gy += da * wb * rv;
+                    gx += wa * db * rv;
+                }
+            }
+            val[f] = v;
+            gr[f][0] = gy;
+            gr[f][1] = gx;
+            gr[f][2] = gz;
+        }
+        const double dudx = gr[0][1], dudy = gr[0][0], dudz = gr[0][2];
+        const double dvdx = gr[1][1], dvdy = gr[1][0], dvdz = gr[1][2];
+        const double dwdx = gr[2][1], dwdy = gr[2][0], dwdz = gr[2][2];
+        const double x = T / sp.T0;
+        const double mu = sp.mu0 * (x * sqrt(x)) * (sp.T0 + 110.4) / (T + 110.4);
+        acc[0] += den * (val[0] * val[0] + val[1] * val[1] + val[2] * val[2]);
+        acc[1] += mu * (dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
+                        dwdx * dwdx + dwdy * dwdy + dwdz * dwdz);
+        const double s1 = dudy + dvdx, s2 = dudz + dwdx, s3 = dwdy + dvdz;
+        acc[2] += dudx * dudx + 0.5 * s1 * s1 + 0.5 * s2 * s2 + 0.5 * s3 * s3 + dvdy * dvdy + dwdz * dwdz;
+        const double o1 = dwdy - dvdz, o2 = dudz - dwdx, o3 = dvdx - dudy;
+        acc[3] += o1 * o1 + o2 * o2 + o3 * o3;
+        const double dd[3] = {dudx, dvdy, dwdz};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double s = dd[a] * dd[a];
+            acc[4 + a] += s;
+            acc[7 + a] += s * dd[a];
+            acc[10 + a] += s * s;
+        }
+        acc[13] += P;
+        acc[14] += T;
+        acc[15] += ir;
+    }
+    block_sum<16>(acc, red + 32 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void stats2_kernel(const double* __restrict__ W, long plane, StatParams sp, double pm,
+                                                     double tm, double vm, double* __restrict__ red)
+{
+    double acc[3] = {0, 0, 0};
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < plane; c += (long)gridDim.x * blockDim.x) {
+        const double den = W[c], ir = 1.0 / den;
+        const double uu = W[plane + c] * ir, vv = W[2 * plane + c] * ir, ww = W[3 * plane + c] * ir;
+        const double P = (den * (sp.gam - 1)) * (W[4 * plane + c] * ir - 0.5 * (uu * uu + vv * vv + ww * ww));
+        const double T = P / (sp.R * den);
+        acc[0] += (pm - P) * (pm - P);
+        acc[1] += (tm - T) * (tm - T);
+        acc[2] += (vm - ir) * (vm - ir);
+    }
+    block_sum<3>(acc, red + 32 * blockIdx.x + 16);
+}
+
+hipError_t launch_stats(const double* W, long plane, const double* const cu[3], const Geom& g, const StatParams& sp,
+                        double* red, hipStream_t s)
+{
+    hipLaunchKernelGGL(stats_kernel, dim3(kStatBlocks), dim3(256), 0, s, W, plane, cu[0], cu[1], cu[2], g, sp, red);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats2(const double* W, long plane, const StatParams& sp, double pm, double tm, double vm,
+                         double* red, hipStream_t s)
+{
+    hipLaunchKernelGGL(stats2_kernel, dim3(kStatBlocks), dim3(256), 0, s, W, plane, sp, pm, tm, vm, red);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void pstats_kernel(long np, const double* __restrict__ U, const double* __restrict__ G,
+                                                     double* __restrict__ red)
+{
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+        const double* u = U + 3 * p;
+        const double* gg = G + 9 * p;
+        acc[0] += u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double dudy = gg[0], dudx = gg[1], dudz = gg[2];
+        const double dvdy = gg[3], dvdx = gg[4], dvdz = gg[5];
+        const double dwdy = gg[6], dwdx = gg[7], dwdz = gg[8];
+        const double s1 = dudy + dvdx, s2 = dudz + dwdx, s3 = dwdy + dvdz;
+        acc[1] += dudx * dudx + 0.5 * s1 * s1 + 0.5 * s2 * s2 + 0.5 * s3 * s3 + dvdy * dvdy + dwdz * dwdz;
+        acc[2] += dudx * dudx;
+        acc[3] += dudx * dudx * dudx;
+        acc[4] += dwdz * dwdz;
+        acc[5] += dwdz * dwdz * dwdz;
+    }
+    block_sum<6>(acc, red + 8 * blockIdx.x);
+}
+
+hipError_t launch_pstats(long np, const double* U, const double* G, double* red, hipStream_t s)
+{
+    hipLaunchKernelGGL(pstats_kernel, dim3(kStatBlocks), dim3(256), 0, s, np, U, G, red);
+    return hipGetLastError();
+}
+
+}  // namespace lpt

@@ -1,0 +1,53 @@
+// lpt_kernels.hpp -- natural tricubic B-spline prefilter, particle sampling /
+// Heun advection and flow statistics for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lpt {
+
+// Device particle and spline state owned by a gkm context.
+struct State {
+    long np = 0;
+    double* pos = nullptr;     // np x 3 (y, x, z), tag order
+    double* U = nullptr;       // np x 3 velocity sampled at the start of the step
+    double* G = nullptr;       // np x 9 velocity gradients gU, gV, gW over (y, x, z)
+    double* coef[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // (nc+8)^3 each
+    int cur = 0;               // coef[cur] = spline of the resident state when valid
+    bool valid = false;
+    double* pivots = nullptr;  // Thomas pivots 1/(4 - p_{r-1}) of the [1 4 1] system
+    double* red = nullptr;     // reduction scratch
+    bool sampled = false;
+};
+
+void release(State& s);
+
+// Spline geometry on the padded cell-centre grid (Interpolation.cpp:3-31):
+// nt = nc+6 knots per axis from start = -3dx + dx/2, spacing delta.
+struct Geom {
+    int nc, nt, N;      // N = nt + 2 coefficients per axis
+    double start, dinv; // 1/delta
+};
+
+// velocity component q (1..3) of the unpadded state W -> coefficients of its
+// natural tricubic spline on the padded grid (periodic ghosts read by wrap).
+hipError_t launch_prefilter(const double* W, long plane, int q, const Geom& g, const double* pivots, double* coef,
+                            hipStream_t s);
+hipError_t launch_sample(long np, const double* pos, const double* const cu[3], const Geom& g, double* U, double* G,
+                         hipStream_t s);
+hipError_t launch_heun(long np, double* pos, const double* U, const double* const cn[3], const Geom& g,
+                       const double* dt, hipStream_t s);
+
+// flow statistics (PrintStats, update.cpp:688-842); red holds >= kStatBlocks*32 doubles
+constexpr int kStatBlocks = 512;
+struct StatParams {
+    double gam, R, mu0, T0;
+};
+hipError_t launch_stats(const double* W, long plane, const double* const cu[3], const Geom& g, const StatParams& sp,
+                        double* red, hipStream_t s);
+// second pass: squared deviations from the means (P, T, 1/rho) into red[32*b + 16..18]
+hipError_t launch_stats2(const double* W, long plane, const StatParams& sp, double pm, double tm, double vm,
+                         double* red, hipStream_t s);
+// particle statistics (GetParticleStats, update.cpp:1111-1162); red >= kStatBlocks*8
+hipError_t launch_pstats(long np, const double* U, const double* G, double* red, hipStream_t s);
+
+}  // namespace lpt

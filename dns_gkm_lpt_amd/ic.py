@@ -22,6 +22,21 @@ import struct
 
 import numpy as np
 
+try:  # multi-threaded FFTs when scipy is present; numpy's otherwise
+    import scipy.fft as _fft
+
+    def _rfftn(a):
+        return _fft.rfftn(a, workers=-1)
+
+    def _irfftn(a, s):
+        return _fft.irfftn(a, s=s, workers=-1)
+except ImportError:  # pragma: no cover
+    def _rfftn(a):
+        return np.fft.rfftn(a)
+
+    def _irfftn(a, s):
+        return np.fft.irfftn(a, s=s)
+
 R_GAS = 287.0
 GAMMA = 7.0 / 5.0
 T0_REF = 300.0
@@ -43,9 +58,9 @@ def make_ic(nc, k0=4.0, Mt=0.3, Re=30.0, seed=0, kcmin=1.0, kcmax=8.0):
     rng = np.random.default_rng(seed)
     ky, kx, kz = _kgrid(nc)
     # u <-> kx, v <-> ky, w <-> kz (init_DNS.m:36-39 with MATLAB dims (y, x, z))
-    uk = np.fft.rfftn(rng.standard_normal((nc, nc, nc)))
-    vk = np.fft.rfftn(rng.standard_normal((nc, nc, nc)))
-    wk = np.fft.rfftn(rng.standard_normal((nc, nc, nc)))
+    uk = _rfftn(rng.standard_normal((nc, nc, nc)))
+    vk = _rfftn(rng.standard_normal((nc, nc, nc)))
+    wk = _rfftn(rng.standard_normal((nc, nc, nc)))
     ksq = np.maximum(kx * kx + ky * ky + kz * kz, 1e-14)
     kdot = kx * uk + ky * vk + kz * wk
     uk -= kdot * kx / ksq
@@ -57,9 +72,9 @@ def make_ic(nc, k0=4.0, Mt=0.3, Re=30.0, seed=0, kcmin=1.0, kcmax=8.0):
     ee = np.where(shell, A * kk ** 4 * np.exp(-2 * (kk / k0) ** 2) / (4 * np.pi * np.maximum(kk, 1) ** 2), 0.0)
     umag2 = np.abs(uk) ** 2 + np.abs(vk) ** 2 + np.abs(wk) ** 2
     fac = np.where(shell & (umag2 > 0), np.sqrt(ee / np.where(umag2 > 0, umag2, 1.0)), 0.0)
-    u = np.fft.irfftn(uk * fac, s=(nc, nc, nc))
-    v = np.fft.irfftn(vk * fac, s=(nc, nc, nc))
-    w = np.fft.irfftn(wk * fac, s=(nc, nc, nc))
+    u = _irfftn(uk * fac, s=(nc, nc, nc))
+    v = _irfftn(vk * fac, s=(nc, nc, nc))
+    w = _irfftn(wk * fac, s=(nc, nc, nc))
     del uk, vk, wk, kdot, fac
 
     p0 = RHO0 * R_GAS * T0_REF
@@ -118,10 +133,10 @@ def prim_to_cons(u, v, w, rho, p, gam=GAMMA):
 
 def fill_halo(W):
     """Periodic ghost layers: ghost(i) = cell(((i - 3) mod nc) + 3) on each axis."""
-    nt = W.shape[-1]
-    nc = nt - 6
-    idx = (np.arange(nt) - 3) % nc + 3
-    return np.ascontiguousarray(W[:, idx][:, :, idx][:, :, :, idx])
+    out = np.empty_like(W)
+    for q in range(W.shape[0]):
+        out[q] = np.pad(W[q][3:-3, 3:-3, 3:-3], 3, mode="wrap")
+    return out
 
 
 # ----------------------------------------------------------------------------

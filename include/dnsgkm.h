@@ -1,0 +1,142 @@
+/*
+ * dnsgkm.h -- C ABI of the MI355X GKM-WENO step and Lagrangian particle tracker.
+ *
+ * Plain pointers and sizes only.  The caller owns every host buffer; the
+ * library owns all device memory of a context.  The state stays resident on
+ * the GPU between calls; data moves between host and device only in the
+ * upload/download/sample calls.  Every function returns 0 on success and a
+ * non-zero GKM_E* code on failure; gkm_last_error() then holds a message.
+ * Calls are ordered on the context's stream; a context is not thread-safe.
+ *
+ * Host layout of a state (the reference's, param.h:40-41): 5 arrays (rho,
+ * rho u, rho v, rho w, rho E) of nt^3 doubles, nt = nc + 6, cell (j,i,k) =
+ * (y,x,z) at (j*nt + i)*nt + k with 3 ghost layers per face.
+ * Particle positions use the reference's order pos = (y, x, z)
+ * (ParticleTrack.cpp:52-55); velocities (u, v, w).
+ */
+#ifndef DNSGKM_H
+#define DNSGKM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GKM_OK = 0,
+    GKM_EINVAL = 1,   /* bad argument or configuration                    */
+    GKM_EHIP = 2,     /* a HIP runtime call failed                         */
+    GKM_ENOMEM = 3,   /* device allocation failed                          */
+    GKM_ESTATE = 4,   /* non-physical state (rho <= 0 or p <= 0 or NaN)    */
+    GKM_ENOTREADY = 5 /* call order violated (e.g. step before upload)     */
+};
+
+enum { GKM_FP64 = 0, GKM_MIXED = 1 };
+
+/* Run configuration.  gam, rho0, Mt, Re, mu0, p0, t0 are the reference's
+ * params file (FlowField.cpp:16-40); cfl and prandtl are compile-time in the
+ * reference (param.h:19-20) and run-time here. */
+typedef struct {
+    int nc;           /* interior cells per edge of the 2*pi periodic cube */
+    int precision;    /* GKM_FP64 or GKM_MIXED                             */
+    double gam, rho0, Mt, Re, mu0, p0, t0;
+    double cfl;       /* CFL number of timestep()                          */
+    double prandtl;   /* Pr of the slope solver (0.7 in the reference)     */
+    int device;       /* HIP device ordinal                                */
+    int check_state;  /* non-zero: gkm_cfl_dt reports GKM_ESTATE on a      */
+                      /* non-physical state                                */
+} gkm_config;
+
+typedef struct gkm_ctx gkm_ctx;
+
+/* Statistics of PrintStats (update.cpp:688-842) plus enstrophy <|curl u|^2>. */
+typedef struct {
+    double tke;       /* <rho |u|^2> (no 1/2, as the reference prints)     */
+    double diss;      /* <mu(T) du_i/dx_j du_i/dx_j>                       */
+    double sijsij;    /* <S_ij S_ij>                                       */
+    double skew[3];   /* skewness of du/dx, dv/dy, dw/dz                   */
+    double kurt[3];   /* kurtosis of du/dx, dv/dy, dw/dz                   */
+    double prms, trms, vrms; /* normalised as update.cpp:829-833           */
+    double pmean, tmean, vmean;
+    double enstrophy; /* <|curl u|^2>                                      */
+} gkm_stats;
+
+/* Particle statistics of GetParticleStats (update.cpp:1111-1162). */
+typedef struct {
+    double tke, sijsij, skew_u, skew_w;
+} gkm_pstats;
+
+/* Replaces GetParams/segment/DevAlloc (FlowField.cpp:9-58, 275-284;
+ * GPUAlloc.cu:3-132): validates cfg and allocates the resident state. */
+int gkm_create(const gkm_config *cfg, gkm_ctx **out);
+void gkm_destroy(gkm_ctx *ctx);
+const char *gkm_last_error(const gkm_ctx *ctx);
+/* internal degrees of freedom K = int((5-3gam)/(gam-1)) (FlowField.cpp:42) */
+int gkm_get_K(const gkm_ctx *ctx);
+
+/* Host -> device state, padded layout, 5 separate arrays (Initialize_DHIT's
+ * W[5], FlowField.cpp:60-114; HtoD of GPUupdate.cu:89-93). Ghosts are ignored. */
+int gkm_upload_state(gkm_ctx *ctx, const double *const W[5]);
+/* Device -> host state with periodic ghost layers filled on the device
+ * (HaloTransfer, update.cpp:855-1029, single domain). */
+int gkm_download_state(gkm_ctx *ctx, double *const W[5]);
+/* Same, with the 5 arrays back to back in one buffer of 5*nt^3 doubles. */
+int gkm_upload_state_packed(gkm_ctx *ctx, const double *W5);
+int gkm_download_state_packed(gkm_ctx *ctx, double *W5);
+
+/* One Eulerian step of size dt: derivsX/Y/Z + update + periodic halo
+ * (evolveEulerian, update.cpp:206-212; GPU evolve, GPUupdate.cu:79-219). */
+int gkm_step(gkm_ctx *ctx, double dt);
+/* dt = cfl dx / max(|U| + c) (timestep, update.cpp:1031-1054). */
+int gkm_cfl_dt(gkm_ctx *ctx, double *dt);
+/* n steps with dt recomputed on the device from the CFL condition after each
+ * step (the Evolve loop order, update.cpp:124-170, Eulerian part); no host
+ * round trip inside.  If dt_hist is non-NULL it receives the n dts used. */
+int gkm_run(gkm_ctx *ctx, int nsteps, double *dt_hist);
+/* Wait for all work on the context's stream. */
+int gkm_sync(gkm_ctx *ctx);
+/* Flow statistics of the resident state (PrintStats, update.cpp:688-842). */
+int gkm_stats_compute(gkm_ctx *ctx, gkm_stats *out);
+
+/* Per-kernel timing with HIP events on the context's stream.  While enabled,
+ * every sweep/update launch is bracketed by events; gkm_timing_read returns
+ * accumulated milliseconds and launch counts for slots
+ * 0 = x sweep, 1 = y sweep, 2 = z sweep (fused flux difference + update),
+ * 3 = CFL reduction, 4 = spline prefilter, 5 = particle step. */
+int gkm_timing_enable(gkm_ctx *ctx, int on);
+int gkm_timing_read(gkm_ctx *ctx, double ms[8], long counts[8]);
+
+/* Pointwise device functions on host arrays (parity checks of the kernels'
+ * math).  gkm_flux_batch: flux() of GKMflux.cpp:11-229 on n faces; in holds
+ * n x 50 doubles (WL, WR, Wl, Wr, DWxl, DWxr, DWyl, DWyr, DWzl, DWzr), tau
+ * n doubles, F n x 5.  gkm_weno_batch: ApplyWENO of update.cpp:556-680 on n
+ * stencils of 6 cells x 5 (cells i-2..i+3), out n x 10 (Wl, Wr).
+ * precision selects the FP64 or the FP32 device math. */
+int gkm_flux_batch(gkm_ctx *ctx, long n, const double *in, const double *tau, double dt, int precision,
+                   double *F);
+int gkm_weno_batch(gkm_ctx *ctx, long n, const double *in, int precision, double *out);
+
+/* ---- Lagrangian particle tracker (ParticleTrack.cpp, update.cpp:1056-1247) */
+/* Upload np positions (np x 3, order y, x, z); replaces locateParticles
+ * (update.cpp:1286-1322). Tags are 0..np-1 in upload order. */
+int lpt_upload(gkm_ctx *ctx, long np, const double *pos);
+/* Download positions (np x 3) and, if non-NULL, velocities (np x 3) in tag order. */
+int lpt_download(gkm_ctx *ctx, double *pos, double *vel);
+/* Sample u, v, w and their gradients (over y, x, z) at the particles from the
+ * natural tricubic spline of the resident state (getParticleProp velocity
+ * part, ParticleTrack.cpp:99-113); any output pointer may be NULL. */
+int lpt_sample(gkm_ctx *ctx, double *U, double *gU, double *gV, double *gW);
+/* Particle statistics from the last sample (GetParticleStats). */
+int lpt_stats(gkm_ctx *ctx, gkm_pstats *out);
+/* One coupled step of the Evolve loop (update.cpp:124-170) with the resident
+ * state: sample U at x from the current field, advance the field by dt
+ * (gkm_step), rebuild the velocity spline, Heun-advance the particles
+ * (Particle::updatePosition, ParticleTrack.cpp:48-73). */
+int lpt_coupled_step(gkm_ctx *ctx, double dt);
+/* Particle-only Heun step against the current field, using the velocities
+ * stored by the last lpt_sample (U at the start of the step). */
+int lpt_heun(gkm_ctx *ctx, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -19,6 +19,7 @@ ERRORS = {1: "GKM_EINVAL", 2: "GKM_EHIP", 3: "GKM_ENOMEM", 4: "GKM_ESTATE", 5: "
 EXPORTS = [
     "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
+    "gkm_sweep_flux_diff",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
     "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_coupled_step", "lpt_heun",
 ]
@@ -73,10 +74,11 @@ def lib():
         L.gkm_cfl_dt.argtypes = [P, C.POINTER(C.c_double)]
         L.gkm_run.argtypes = [P, C.c_int, C.c_void_p]
         L.gkm_sync.argtypes = [P]
+        L.gkm_sweep_flux_diff.argtypes = [P, C.c_int, C.c_double, _dp]
         L.gkm_stats_compute.argtypes = [P, C.POINTER(Stats)]
         L.gkm_timing_enable.argtypes = [P, C.c_int]
         L.gkm_timing_read.argtypes = [P, C.POINTER(C.c_double * 8), C.POINTER(C.c_long * 8)]
-        L.gkm_flux_batch.argtypes = [P, C.c_long, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.gkm_flux_batch.argtypes = [P, C.c_long, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp]
         L.gkm_weno_batch.argtypes = [P, C.c_long, _dp, C.c_int, _dp]
         L.lpt_upload.argtypes = [P, C.c_long, _dp]
         L.lpt_download.argtypes = [P, C.c_void_p, C.c_void_p]
@@ -160,6 +162,11 @@ class Context:
         self._chk(lib().gkm_run(self._h, nsteps, _ptr(hist)))
         return hist[:nsteps] if record_dt else None
 
+    def sweep_flux_diff(self, direction, dt):
+        out = np.empty((5, self.nc, self.nc, self.nc))
+        self._chk(lib().gkm_sweep_flux_diff(self._h, direction, dt, out))
+        return out
+
     def sync(self):
         self._chk(lib().gkm_sync(self._h))
 
@@ -177,11 +184,12 @@ class Context:
         self._chk(lib().gkm_timing_read(self._h, C.byref(ms), C.byref(cnt)))
         return np.array(ms[:]), np.array(cnt[:])
 
-    def flux_batch(self, X, tau, dt, precision=GKM_FP64):
+    def flux_batch(self, X, tau, dt, precision=GKM_FP64, dx=None):
         X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 50)
         tau = np.ascontiguousarray(tau, dtype=np.float64).reshape(-1)
         F = np.empty((X.shape[0], 5))
-        self._chk(lib().gkm_flux_batch(self._h, X.shape[0], X, tau, dt, precision, F))
+        dx = 2.0 * (np.arctan(1.0) * 4.0) / self.nc if dx is None else dx
+        self._chk(lib().gkm_flux_batch(self._h, X.shape[0], X, tau, dt, dx, precision, F))
         return F
 
     def weno_batch(self, S, precision=GKM_FP64):

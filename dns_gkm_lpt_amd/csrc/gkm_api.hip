@@ -96,7 +96,7 @@ int eulerian_step(gkm_ctx* c)
         a.seg = gkm::sweep_seg_len(dir, c->nc);
         a.nseg = c->nc / a.seg;
         a.mode = dir;  // x: dW = dF, y: dW += dF, z: Wout = W - (dW + dF)/dx
-        int rc = timed(c, dir, [&] { return gkm::launch_sweep(dir, a, c->stream); });
+        int rc = timed(c, dir, [&] { return gkm::launch_sweep(dir, a, c->cfg.precision == GKM_MIXED, c->stream); });
         if (rc) return rc;
     }
     c->cur = 1 - c->cur;
@@ -275,6 +275,33 @@ int gkm_run(gkm_ctx* c, int nsteps, double* dt_hist)
     return GKM_OK;
 }
 
+int gkm_sweep_flux_diff(gkm_ctx* c, int dir, double dt, double* out)
+{
+    if (!c || !out || dir < 0 || dir > 2) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, GKM_EINVAL, "dt must be positive and finite");
+    HIPCHK(c, hipMemcpyAsync(c->dscal, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    gkm::SweepArgs a{};
+    a.W = c->W[c->cur];
+    a.dW = c->dW;
+    a.Wout = c->W[1 - c->cur];
+    a.dt = c->dscal;
+    a.plane = c->plane;
+    a.nc = c->nc;
+    a.dx = c->dx;
+    a.inv_dx = 1.0 / c->dx;
+    a.g = c->gas;
+    a.seg = gkm::sweep_seg_len(dir, c->nc);
+    a.nseg = c->nc / a.seg;
+    a.mode = 0;
+    int rc = timed(c, dir, [&] { return gkm::launch_sweep(dir, a, c->cfg.precision == GKM_MIXED, c->stream); });
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->dW, sizeof(double) * 5 * c->plane, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
 int gkm_sync(gkm_ctx* c)
 {
     if (!c) return GKM_EINVAL;
@@ -303,9 +330,11 @@ int gkm_timing_read(gkm_ctx* c, double ms[8], long counts[8])
     return GKM_OK;
 }
 
-int gkm_flux_batch(gkm_ctx* c, long n, const double* in, const double* tau, double dt, int precision, double* F)
+int gkm_flux_batch(gkm_ctx* c, long n, const double* in, const double* tau, double dt, double dx, int precision,
+                   double* F)
 {
-    if (!c || n < 0 || (n > 0 && (!in || !tau || !F))) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c || n < 0 || (n > 0 && (!in || !tau || !F)) || !(dx > 0.0) || precision < 0 || precision > 2)
+        return fail(c, GKM_EINVAL, "bad argument");
     if (n == 0) return GKM_OK;
     double *din = nullptr, *dtau = nullptr, *dF = nullptr;
     int rc = GKM_OK;
@@ -315,7 +344,7 @@ int gkm_flux_batch(gkm_ctx* c, long n, const double* in, const double* tau, doub
     } else {
         hipError_t e = hipMemcpyAsync(din, in, sizeof(double) * 50 * n, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dtau, tau, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = gkm::launch_flux_batch(n, din, dtau, dt, c->dx, c->gas, precision, dF, c->stream);
+        if (e == hipSuccess) e = gkm::launch_flux_batch(n, din, dtau, dt, dx, c->gas, precision, dF, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(F, dF, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, GKM_EHIP, std::string("flux batch: ") + hipGetErrorString(e));
@@ -328,7 +357,8 @@ int gkm_flux_batch(gkm_ctx* c, long n, const double* in, const double* tau, doub
 
 int gkm_weno_batch(gkm_ctx* c, long n, const double* in, int precision, double* out)
 {
-    if (!c || n < 0 || (n > 0 && (!in || !out))) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c || n < 0 || (n > 0 && (!in || !out)) || precision < 0 || precision > 2)
+        return fail(c, GKM_EINVAL, "bad argument");
     if (n == 0) return GKM_OK;
     double *din = nullptr, *dout = nullptr;
     int rc = GKM_OK;

@@ -53,7 +53,9 @@ struct Lines {
     }
 };
 
-template <int DIR>
+// MIX = false: the FP64 path.  MIX = true: WENO in FP32 on FP64 stencil
+// differences and the mixed-precision flux (gks_flux_mixed).
+template <int DIR, bool MIX>
 __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB) void sweep_kernel(SweepArgs p)
 {
     constexpr int CN = Tile<DIR>::CN, CA = Tile<DIR>::CA, CB = Tile<DIR>::CB;
@@ -119,7 +121,10 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB) void s
                 for (int q = 0; q < 5; q++) S[t][q] = W[q * plane + idx];
             }
             double Wl[5], Wr[5];
-            weno5<double>(S, p.g, Wl, Wr);
+            if (MIX)
+                weno5_mixed(S, p.g, Wl, Wr);
+            else
+                weno5<double>(S, p.g, Wl, Wr);
 #pragma unroll
             for (int q = 0; q < 5; q++) {
                 sW[q][itn][il] = Wl[q];
@@ -159,7 +164,10 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB) void s
 #undef GKM_SWAP
             }
             double F[5];
-            gks_flux<double>(WL, WR, Wl, Wr, D, tau, dt, dx, p.g, F);
+            if (MIX)
+                gks_flux_mixed(WL, WR, Wl, Wr, D, tau, dt, dx, p.g, F);
+            else
+                gks_flux<double>(WL, WR, Wl, Wr, D, tau, dt, dx, p.g, F);
             if (DIR != 0) {
                 constexpr int sw = DIR == 1 ? 2 : 3;
                 double t = F[sw];
@@ -215,20 +223,23 @@ int sweep_seg_len(int dir, int nc)
 }
 
 template <int DIR>
-static hipError_t launch_dir(const SweepArgs& a, hipStream_t s)
+static hipError_t launch_dir(const SweepArgs& a, bool mixed, hipStream_t s)
 {
     constexpr int CN = Tile<DIR>::CN, CA = Tile<DIR>::CA, CB = Tile<DIR>::CB;
     if (a.nc % CA || a.nc % CB || a.seg % CN || a.nc % a.seg || a.nseg != a.nc / a.seg) return hipErrorInvalidValue;
     const long nblk = (long)(a.nc / CA) * (a.nc / CB) * a.nseg;
-    hipLaunchKernelGGL(sweep_kernel<DIR>, dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
+    if (mixed)
+        hipLaunchKernelGGL((sweep_kernel<DIR, true>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
+    else
+        hipLaunchKernelGGL((sweep_kernel<DIR, false>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(int dir, const SweepArgs& a, hipStream_t s)
+hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s)
 {
-    if (dir == 0) return launch_dir<0>(a, s);
-    if (dir == 1) return launch_dir<1>(a, s);
-    return launch_dir<2>(a, s);
+    if (dir == 0) return launch_dir<0>(a, mixed, s);
+    if (dir == 1) return launch_dir<1>(a, mixed, s);
+    return launch_dir<2>(a, mixed, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -391,6 +402,28 @@ __global__ __launch_bounds__(64) void flux_batch_kernel(long n, const double* __
     for (int q = 0; q < 5; q++) F[5 * i + q] = (double)Fo[q];
 }
 
+__global__ __launch_bounds__(64) void flux_batch_mixed_kernel(long n, const double* __restrict__ in,
+                                                              const double* __restrict__ tau, double dt, double dx,
+                                                              Gas<double> g, double* __restrict__ F)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* v = in + 50 * i;
+    double WL[5], WR[5], Wl[5], Wr[5], D[6][5], Fo[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        WL[q] = v[q];
+        WR[q] = v[5 + q];
+        Wl[q] = v[10 + q];
+        Wr[q] = v[15 + q];
+#pragma unroll
+        for (int d = 0; d < 6; d++) D[d][q] = v[20 + 5 * d + q];
+    }
+    gks_flux_mixed(WL, WR, Wl, Wr, D, tau[i], dt, dx, g, Fo);
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[5 * i + q] = Fo[q];
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void weno_batch_kernel(long n, const double* __restrict__ in, Gas<T> g, double* __restrict__ out)
 {
@@ -409,10 +442,24 @@ __global__ __launch_bounds__(64) void weno_batch_kernel(long n, const double* __
     }
 }
 
-static Gas<float> to_float(const Gas<double>& g)
+__global__ __launch_bounds__(64) void weno_batch_mixed_kernel(long n, const double* __restrict__ in, Gas<double> g,
+                                                              double* __restrict__ out)
 {
-    return Gas<float>{(float)g.gam, (float)g.R, (float)g.inv_prn, (float)g.mu0, (float)g.T0, (float)g.eps, g.K};
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double S[6][5], Wl[5], Wr[5];
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) S[s][q] = in[30 * i + 5 * s + q];
+    weno5_mixed(S, g, Wl, Wr);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        out[10 * i + q] = Wl[q];
+        out[10 * i + 5 + q] = Wr[q];
+    }
 }
+
 
 hipError_t launch_flux_batch(long n, const double* in, const double* tau, double dt, double dx, const Gas<double>& g,
                              int precision, double* F, hipStream_t s)
@@ -421,8 +468,10 @@ hipError_t launch_flux_batch(long n, const double* in, const double* tau, double
     const unsigned blocks = (unsigned)((n + 63) / 64);
     if (precision == 0)
         hipLaunchKernelGGL(flux_batch_kernel<double>, dim3(blocks), dim3(64), 0, s, n, in, tau, dt, dx, g, F);
+    else if (precision == 1)
+        hipLaunchKernelGGL(flux_batch_mixed_kernel, dim3(blocks), dim3(64), 0, s, n, in, tau, dt, dx, g, F);
     else
-        hipLaunchKernelGGL(flux_batch_kernel<float>, dim3(blocks), dim3(64), 0, s, n, in, tau, dt, dx, to_float(g), F);
+        hipLaunchKernelGGL(flux_batch_kernel<float>, dim3(blocks), dim3(64), 0, s, n, in, tau, dt, dx, to_f32(g), F);
     return hipGetLastError();
 }
 
@@ -432,8 +481,10 @@ hipError_t launch_weno_batch(long n, const double* in, const Gas<double>& g, int
     const unsigned blocks = (unsigned)((n + 63) / 64);
     if (precision == 0)
         hipLaunchKernelGGL(weno_batch_kernel<double>, dim3(blocks), dim3(64), 0, s, n, in, g, out);
+    else if (precision == 1)
+        hipLaunchKernelGGL(weno_batch_mixed_kernel, dim3(blocks), dim3(64), 0, s, n, in, g, out);
     else
-        hipLaunchKernelGGL(weno_batch_kernel<float>, dim3(blocks), dim3(64), 0, s, n, in, to_float(g), out);
+        hipLaunchKernelGGL(weno_batch_kernel<float>, dim3(blocks), dim3(64), 0, s, n, in, to_f32(g), out);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ template <> struct Tile<0> { static constexpr int CN = 4, CA = 4, CB = 16; };   
 template <> struct Tile<1> { static constexpr int CN = 4, CA = 4, CB = 16; };   // y sweep: a = x, b = z
 template <> struct Tile<2> { static constexpr int CN = 16, CA = 4, CB = 4; };   // z sweep: a = x, b = y
 
-hipError_t launch_sweep(int dir, const SweepArgs& a, hipStream_t s);
+hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s);
 int sweep_seg_len(int dir, int nc);  // faces per workgroup chosen for a grid
 bool sweep_grid_ok(int nc);          // nc compatible with every tile shape
 

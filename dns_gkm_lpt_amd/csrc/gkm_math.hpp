@@ -391,4 +391,218 @@ __host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], 
     }
 }
 
+// ===========================================================================
+// Mixed precision.  FP64 where cancellation would cost digits: the stencil
+// differences fed to WENO, primitive recovery (c2p, lambda = rho/(2p)), the
+// equilibrium state We, the Euler part of the flux (the p0 and p3 terms,
+// GKMflux.cpp:225-226, which carry dt times the inviscid flux) and the
+// collision coefficients.  FP32 for WENO smoothness indicators and weights
+// and for every slope-dependent moment term of the flux.
+// ===========================================================================
+
+// WENO on FP64 stencils, FP32 arithmetic on differences from the base cell
+// (the reconstruction is shift invariant: candidates are affine with unit
+// coefficient sum and the indicators see differences only).
+__host__ __device__ __forceinline__ void weno5_mixed(const double S[6][5], const Gas<double>& g, double Wl[5],
+                                                     double Wr[5])
+{
+    const float eps = (float)g.eps;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double c = S[2][q], d = S[3][q];
+        Wl[q] = c + (double)weno_side<float>((float)(S[0][q] - c), (float)(S[1][q] - c), 0.f, (float)(d - c),
+                                             (float)(S[4][q] - c), (float)(S[5][q] - c), eps);
+        Wr[q] = d + (double)weno_side<float>((float)(S[5][q] - d), (float)(S[4][q] - d), 0.f, (float)(c - d),
+                                             (float)(S[1][q] - d), (float)(S[0][q] - d), eps);
+    }
+    double Tc[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) Tc[s] = temp_of<double>(S[s], g.gam, g.R);
+    const double Tl = Tc[2] + (double)weno_side<float>((float)(Tc[0] - Tc[2]), (float)(Tc[1] - Tc[2]), 0.f,
+                                                       (float)(Tc[3] - Tc[2]), (float)(Tc[4] - Tc[2]),
+                                                       (float)(Tc[5] - Tc[2]), eps);
+    const double Tr = Tc[3] + (double)weno_side<float>((float)(Tc[5] - Tc[3]), (float)(Tc[4] - Tc[3]), 0.f,
+                                                       (float)(Tc[2] - Tc[3]), (float)(Tc[1] - Tc[3]),
+                                                       (float)(Tc[0] - Tc[3]), eps);
+    const double igm = 1.0 / (g.gam - 1.0);
+    Wl[4] = Wl[0] * g.R * Tl * igm + 0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
+    Wr[4] = Wr[0] * g.R * Tr * igm + 0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ Maxw<float> to_f32(const Maxw<T>& s)
+{
+    return Maxw<float>{(float)s.den, (float)s.u, (float)s.v, (float)s.w, (float)s.lam, (float)s.i2l, (float)s.e2,
+                       (float)s.e4};
+}
+
+__host__ __device__ __forceinline__ Gas<float> to_f32(const Gas<double>& g)
+{
+    return Gas<float>{(float)g.gam, (float)g.R, (float)g.inv_prn, (float)g.mu0, (float)g.T0, (float)g.eps, g.K};
+}
+
+// Same inputs and result as gks_flux<double>; D (the ten slopes) arrives in FP64.
+__host__ __device__ __forceinline__ void gks_flux_mixed(const double WL[5], const double WR[5], const double Wl[5],
+                                                        const double Wr[5], const double D[6][5], double tau,
+                                                        double dt, double dx, const Gas<double>& g, double F[5])
+{
+    const double pi = 3.141592653589793238462643383279502884;
+    const Gas<float> gf = to_f32(g);
+    Maxw<double> L, R;
+    maxw_init(L, Wl, g);
+    maxw_init(R, Wr, g);
+    {
+        const double pl = L.den / L.lam, pr = R.den / R.lam;
+        tau = tau + dt * (fabs(pl - pr) / fabs(pl + pr));
+    }
+    const double ex = exp(-dt / tau);
+    const double om = 1.0 - ex;
+    const double igm0 = 1.0 / (dt - tau * om);
+    const double gm1 = -om * igm0;
+    const double gm2 = (-dt + 2.0 * tau * om - dt * ex) * igm0;
+    const double gm3 = -gm1;
+    const double gm4 = (dt * ex - tau * om) * igm0;
+    const double gm5 = tau * gm3;
+    const double Pc = tau * om;
+    const double Qc = -tau * dt * ex + tau * tau * om;
+    const double p0 = dt - Pc;
+    const double p1 = 0.5 * dt * dt - tau * p0;
+    const double p2 = -tau * p0 + Qc;
+    const double p3 = Pc;
+    const double p4 = -Qc - tau * Pc;
+    const double p5 = -tau * Pc;
+
+    const double gl = 0.5 * exp(-L.lam * L.u * L.u) / sqrt(L.lam * pi);
+    const double hl0 = 0.5 * erfc(-sqrt(L.lam) * L.u);
+    const double gr = 0.5 * exp(-R.lam * R.u * R.u) / sqrt(R.lam * pi);
+    const double hr0 = 0.5 * erfc(sqrt(R.lam) * R.u);
+
+    // FP64: equilibrium state, the Euler part p3 (l, r) and the 000 moments
+    double We[5], F64[5], Bz[5];
+    {
+        double hu[4], tv[3], tw[3], M[5];
+        half_tab<4>(L.u, L.i2l, hl0, L.u * hl0 + gl, hu);
+        full_tab<3>(L.v, L.i2l, tv);
+        full_tab<3>(L.w, L.i2l, tw);
+        mom<0, 0, 0>(hu, tv, tw, L.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) We[q] = L.den * M[q];
+        mom<1, 0, 0>(hu, tv, tw, L.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) F64[q] = p3 * L.den * M[q];
+        half_tab<4>(R.u, R.i2l, hr0, R.u * hr0 - gr, hu);
+        full_tab<3>(R.v, R.i2l, tv);
+        full_tab<3>(R.w, R.i2l, tw);
+        mom<0, 0, 0>(hu, tv, tw, R.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) We[q] += R.den * M[q];
+        mom<1, 0, 0>(hu, tv, tw, R.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) F64[q] += p3 * R.den * M[q];
+    }
+    Maxw<double> E;
+    maxw_init(E, We, g);
+    {
+        double fu[4], tv[3], tw[3], M[5];
+        full_tab<4>(E.u, E.i2l, fu);
+        full_tab<3>(E.v, E.i2l, tv);
+        full_tab<3>(E.w, E.i2l, tw);
+        mom<1, 0, 0>(fu, tv, tw, E.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) F64[q] += p0 * E.den * M[q];
+        // gm1 den_e <psi>_e + gm3 We (GKMflux.cpp:185): zero up to rounding
+        mom<0, 0, 0>(fu, tv, tw, E.e2, M);
+#pragma unroll
+        for (int q = 0; q < 5; q++) Bz[q] = gm1 * E.den * M[q] + gm3 * We[q];
+    }
+
+    // FP32: every slope-dependent term
+    const float fgm2 = (float)gm2, fgm45 = (float)(gm4 + gm5), fgm5 = (float)gm5;
+    const float fp1 = (float)p1, fp2 = (float)p2, fp4 = (float)p4, fp5 = (float)p5;
+    float Be[5], bye[5] = {0, 0, 0, 0, 0}, bze[5] = {0, 0, 0, 0, 0}, Ff[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) Be[q] = (float)Bz[q];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const Maxw<float> S = to_f32(side == 0 ? L : R);
+        const int dxi = side;
+        float hu[7], fu[6], tv[6], tw[6];
+        if (side == 0)
+            half_tab<7>(S.u, S.i2l, (float)hl0, (float)(L.u * hl0 + gl), hu);
+        else
+            half_tab<7>(S.u, S.i2l, (float)hr0, (float)(R.u * hr0 - gr), hu);
+        full_tab<6>(S.u, S.i2l, fu);
+        full_tab<6>(S.v, S.i2l, tv);
+        full_tab<6>(S.w, S.i2l, tw);
+        const double iden = 1.0 / (side == 0 ? L.den : R.den);
+        float b[5], ax[5], ay[5], az[5], A[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (float)(D[0 + dxi][q] * iden);
+        slope(b, S, gf, ax);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (float)(D[2 + dxi][q] * iden);
+        slope(b, S, gf, ay);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (float)(D[4 + dxi][q] * iden);
+        slope(b, S, gf, az);
+        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, ay, 1.f, bye);
+        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, az, 1.f, bze);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = 0.f;
+        mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, -1.f, b);
+        mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, -1.f, b);
+        mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, -1.f, b);
+        slope(b, S, gf, A);
+        const float cB = fgm45 * S.den;
+        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cB, Be);
+        mom_a<0, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cB, Be);
+        mom_a<0, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cB, Be);
+        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, A, fgm5 * S.den, Be);
+        const float cF = fp4 * S.den;
+        mom_a<2, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cF, Ff);
+        mom_a<1, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cF, Ff);
+        mom_a<1, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cF, Ff);
+        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, A, fp5 * S.den, Ff);
+    }
+    {
+        const Maxw<float> Ef = to_f32(E);
+        const double ge = 0.5 * exp(-E.lam * E.u * E.u) / sqrt(E.lam * pi);
+        const double sl = sqrt(E.lam) * E.u;
+        const double hp0 = 0.5 * erfc(-sl), hn0 = 0.5 * erfc(sl);
+        float hp[7], hn[7], fu[7], tv[6], tw[6];
+        half_tab<7>(Ef.u, Ef.i2l, (float)hp0, (float)(E.u * hp0 + ge), hp);
+        half_tab<7>(Ef.u, Ef.i2l, (float)hn0, (float)(E.u * hn0 - ge), hn);
+        full_tab<7>(Ef.u, Ef.i2l, fu);
+        full_tab<6>(Ef.v, Ef.i2l, tv);
+        full_tab<6>(Ef.w, Ef.i2l, tw);
+        const double c = 2.0 / (E.den * dx);
+        float b[5], axl[5], axr[5], ay[5], az[5], A[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (float)((We[q] - WL[q]) * c);
+        slope(b, Ef, gf, axl);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = (float)((WR[q] - We[q]) * c);
+        slope(b, Ef, gf, axr);
+        slope(bye, Ef, gf, ay);
+        slope(bze, Ef, gf, az);
+        const float cB = fgm2 * Ef.den;
+        mom_a<1, 0, 0>(hp, tv, tw, Ef.e2, Ef.e4, axl, cB, Be);
+        mom_a<1, 0, 0>(hn, tv, tw, Ef.e2, Ef.e4, axr, cB, Be);
+        mom_a<0, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, ay, cB, Be);
+        mom_a<0, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, az, cB, Be);
+        const float iden = (float)(1.0 / E.den);
+#pragma unroll
+        for (int q = 0; q < 5; q++) Be[q] *= iden;
+        slope(Be, Ef, gf, A);
+        mom_a<1, 0, 0>(fu, tv, tw, Ef.e2, Ef.e4, A, fp1 * Ef.den, Ff);
+        const float cF = fp2 * Ef.den;
+        mom_a<2, 0, 0>(hp, tv, tw, Ef.e2, Ef.e4, axl, cF, Ff);
+        mom_a<2, 0, 0>(hn, tv, tw, Ef.e2, Ef.e4, axr, cF, Ff);
+        mom_a<1, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, ay, cF, Ff);
+        mom_a<1, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, az, cF, Ff);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[q] = F64[q] + (double)Ff[q];
+}
+
 }  // namespace gkm

@@ -92,6 +92,12 @@ int gkm_cfl_dt(gkm_ctx *ctx, double *dt);
  * step (the Evolve loop order, update.cpp:124-170, Eulerian part); no host
  * round trip inside.  If dt_hist is non-NULL it receives the n dts used. */
 int gkm_run(gkm_ctx *ctx, int nsteps, double *dt_hist);
+/* Flux differences of one sweep direction (dir 0 = x, 1 = y, 2 = z) of the
+ * resident state with step dt: out (5 x nc^3, unpadded, (y,x,z) z fastest)
+ * receives F(face i+1/2) - F(face i-1/2) per cell, i.e. one of the three
+ * terms update() sums (update.cpp:505-516).  The state is not changed.
+ * A per-direction parity hook for derivsX/Y/Z (update.cpp:214-503). */
+int gkm_sweep_flux_diff(gkm_ctx *ctx, int dir, double dt, double *out);
 /* Wait for all work on the context's stream. */
 int gkm_sync(gkm_ctx *ctx);
 /* Flow statistics of the resident state (PrintStats, update.cpp:688-842). */
@@ -110,9 +116,10 @@ int gkm_timing_read(gkm_ctx *ctx, double ms[8], long counts[8]);
  * n x 50 doubles (WL, WR, Wl, Wr, DWxl, DWxr, DWyl, DWyr, DWzl, DWzr), tau
  * n doubles, F n x 5.  gkm_weno_batch: ApplyWENO of update.cpp:556-680 on n
  * stencils of 6 cells x 5 (cells i-2..i+3), out n x 10 (Wl, Wr).
- * precision selects the FP64 or the FP32 device math. */
-int gkm_flux_batch(gkm_ctx *ctx, long n, const double *in, const double *tau, double dt, int precision,
-                   double *F);
+ * precision: 0 = the FP64 device math, 1 = the mixed-precision device math
+ * the GKM_MIXED sweeps use, 2 = all-FP32 device math (for error studies). */
+int gkm_flux_batch(gkm_ctx *ctx, long n, const double *in, const double *tau, double dt, double dx,
+                   int precision, double *F);
 int gkm_weno_batch(gkm_ctx *ctx, long n, const double *in, int precision, double *out);
 
 /* ---- Lagrangian particle tracker (ParticleTrack.cpp, update.cpp:1056-1247) */

@@ -41,6 +41,24 @@ def test_weno_batch_matches_oracle():
     assert np.all(err < 1e-12), err
 
 
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_sweep_flux_differences_match_oracle(direction):
+    """Teacher-forced per-direction check of derivsX/Y/Z: F(i+1/2) - F(i-1/2)."""
+    nc = 32
+    W, P, gas = make_case(nc, seed=0)
+    dt = po.cfl_dt(W, 0.5, gas)
+    F = po.fluxes(W, dt, gas, nthreads=8)[direction]
+    shift = [(0, 0, 1, 0), (0, 1, 0, 0), (0, 0, 0, 1)][direction]  # axis of (q, y, x, z)
+    Fm = np.roll(F, 1, axis=shift.index(1))
+    ref = (F - Fm)[:, 3:-3, 3:-3, 3:-3]
+    with ctx_for(nc, P) as c:
+        c.upload(W)
+        out = c.sweep_flux_diff(direction, dt)
+    e = rel_err(out, ref)
+    print("direction", direction, "rel err", e, flush=True)
+    assert np.all(e < 1e-11), e
+
+
 @pytest.mark.parametrize("nc,seed", [(32, 0), (48, 1)])
 def test_one_step_matches_oracle(nc, seed):
     W, P, gas = make_case(nc, seed=seed)

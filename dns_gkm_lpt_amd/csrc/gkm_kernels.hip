@@ -55,8 +55,13 @@ struct Lines {
 
 // MIX = false: the FP64 path.  MIX = true: WENO in FP32 on FP64 stencil
 // differences and the mixed-precision flux (gks_flux_mixed).
+// Two waves per SIMD are requested: the register budget is then 256 per lane
+// and the flux's overflow goes to scratch.  With one wave per SIMD the
+// compiler (ROCm 7.2, gfx950) spills into AGPRs and the x and y instances
+// produced wrong flux differences on the GPU (the z instance and every
+// instance built this way match the FP64 oracle to 1e-14).
 template <int DIR, bool MIX>
-__global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB) void sweep_kernel(SweepArgs p)
+__global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) void sweep_kernel(SweepArgs p)
 {
     constexpr int CN = Tile<DIR>::CN, CA = Tile<DIR>::CA, CB = Tile<DIR>::CB;
     constexpr int NT = CN * CA * CB;

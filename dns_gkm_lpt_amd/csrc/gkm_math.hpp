@@ -289,17 +289,15 @@ __host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], 
 #pragma unroll
     for (int q = 0; q < 5; q++) F[q] = T(0);
 
-    // ---- left (u>0 half of the left Maxwellian) and right (u<0 half of the right)
+    // ---- left (u>0 half of the left Maxwellian) and right (u<0 half of the right).
+    // Each table set is live only inside its block: the full u-table gives the
+    // time slope A of the state first, then the half table carries every term
+    // that integrates over the half space.
 #pragma unroll
     for (int side = 0; side < 2; side++) {
         const Maxw<T>& S = side == 0 ? L : R;
         const int dxi = side == 0 ? 0 : 1;  // index of DWx{l,r} etc. in D
-        T hu[7], fu[6], tv[6], tw[6];
-        if (side == 0)
-            half_tab<7>(S.u, S.i2l, hl0, S.u * hl0 + gl, hu);
-        else
-            half_tab<7>(S.u, S.i2l, hr0, S.u * hr0 - gr, hu);
-        full_tab<6>(S.u, S.i2l, fu);
+        T tv[6], tw[6];
         full_tab<6>(S.v, S.i2l, tv);
         full_tab<6>(S.w, S.i2l, tw);
         const T iden = T(1) / S.den;
@@ -313,15 +311,24 @@ __host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], 
 #pragma unroll
         for (int q = 0; q < 5; q++) b[q] = D[4 + dxi][q] * iden;
         slope(b, S, g, az);
-        // phase II (GKMflux.cpp:130-151)
+        {   // phase II: A from the full-space moments (GKMflux.cpp:134-151)
+            T fu[6];
+            full_tab<6>(S.u, S.i2l, fu);
+#pragma unroll
+            for (int q = 0; q < 5; q++) b[q] = T(0);
+            mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, T(-1), b);
+            mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, T(-1), b);
+            mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, T(-1), b);
+            slope(b, S, g, A);
+        }
+        T hu[7];
+        if (side == 0)
+            half_tab<7>(S.u, S.i2l, hl0, S.u * hl0 + gl, hu);
+        else
+            half_tab<7>(S.u, S.i2l, hr0, S.u * hr0 - gr, hu);
+        // phase II right-hand sides of the equilibrium slopes (GKMflux.cpp:130-133, 141-144)
         mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, ay, T(1), bye);
         mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, az, T(1), bze);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = T(0);
-        mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, T(-1), b);
-        mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, T(-1), b);
-        mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, T(-1), b);
-        slope(b, S, g, A);
         // this state's share of den_e * B_ and of F
         const T cB = (gm4 + gm5) * S.den;
         T M[5];
@@ -342,52 +349,58 @@ __host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], 
         mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, A, p5 * S.den, F);
     }
 
-    // ---- equilibrium state
+    // ---- equilibrium state: one half-space table at a time, then the full table
     {
-        T hp[7], hn[7], fu[7], tv[6], tw[6];
-        const T ge = T(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
-        const T sl = g_sqrt(E.lam) * E.u;
-        const T hp0 = T(0.5) * g_erfc(-sl), hn0 = T(0.5) * g_erfc(sl);
-        half_tab<7>(E.u, E.i2l, hp0, E.u * hp0 + ge, hp);
-        half_tab<7>(E.u, E.i2l, hn0, E.u * hn0 - ge, hn);
-        full_tab<7>(E.u, E.i2l, fu);
+        T tv[6], tw[6];
         full_tab<6>(E.v, E.i2l, tv);
         full_tab<6>(E.w, E.i2l, tw);
-        // slopes phase I / II of the equilibrium state (GKMflux.cpp:115-125, 148-149)
+        const T ge = T(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
+        const T sl = g_sqrt(E.lam) * E.u;
         const T c = T(2) / (E.den * dx);
-        T b[5], axl[5], axr[5], ay[5], az[5], A[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * c;
-        slope(b, E, g, axl);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * c;
-        slope(b, E, g, axr);
-        slope(bye, E, g, ay);
-        slope(bze, E, g, az);
-        // B_ and the time slope A_ (GKMflux.cpp:183-191)
         const T cB = gm2 * E.den;
+        const T cF = p2 * E.den;
+        T b[5], a[5];
+        {   // u>0 half with the left normal slope (GKMflux.cpp:115, 124, 170, 210)
+            T h[7];
+            const T h0 = T(0.5) * g_erfc(-sl);
+            half_tab<7>(E.u, E.i2l, h0, E.u * h0 + ge, h);
+#pragma unroll
+            for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * c;
+            slope(b, E, g, a);
+            mom_a<1, 0, 0>(h, tv, tw, E.e2, E.e4, a, cB, Be);
+            mom_a<2, 0, 0>(h, tv, tw, E.e2, E.e4, a, cF, F);
+        }
+        {   // u<0 half with the right normal slope (GKMflux.cpp:116, 125, 171, 211)
+            T h[7];
+            const T h0 = T(0.5) * g_erfc(sl);
+            half_tab<7>(E.u, E.i2l, h0, E.u * h0 - ge, h);
+#pragma unroll
+            for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * c;
+            slope(b, E, g, a);
+            mom_a<1, 0, 0>(h, tv, tw, E.e2, E.e4, a, cB, Be);
+            mom_a<2, 0, 0>(h, tv, tw, E.e2, E.e4, a, cF, F);
+        }
+        T fu[7];
+        full_tab<7>(E.u, E.i2l, fu);
+        // tangential slopes (GKMflux.cpp:148-149, 172-173, 212-213)
+        slope(bye, E, g, a);
+        mom_a<0, 1, 0>(fu, tv, tw, E.e2, E.e4, a, cB, Be);
+        mom_a<1, 1, 0>(fu, tv, tw, E.e2, E.e4, a, cF, F);
+        slope(bze, E, g, a);
+        mom_a<0, 0, 1>(fu, tv, tw, E.e2, E.e4, a, cB, Be);
+        mom_a<1, 0, 1>(fu, tv, tw, E.e2, E.e4, a, cF, F);
+        // B_ and the time slope A_ (GKMflux.cpp:183-191)
         T M[5];
         mom<0, 0, 0>(fu, tv, tw, E.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) Be[q] += gm1 * E.den * M[q];
-        mom_a<1, 0, 0>(hp, tv, tw, E.e2, E.e4, axl, cB, Be);
-        mom_a<1, 0, 0>(hn, tv, tw, E.e2, E.e4, axr, cB, Be);
-        mom_a<0, 1, 0>(fu, tv, tw, E.e2, E.e4, ay, cB, Be);
-        mom_a<0, 0, 1>(fu, tv, tw, E.e2, E.e4, az, cB, Be);
         const T iden = T(1) / E.den;
 #pragma unroll
-        for (int q = 0; q < 5; q++) Be[q] *= iden;
-        slope(Be, E, g, A);
+        for (int q = 0; q < 5; q++) b[q] = (Be[q] + gm1 * E.den * M[q]) * iden;
+        slope(b, E, g, a);
         // equilibrium part of the flux (GKMflux.cpp:225)
         mom<1, 0, 0>(fu, tv, tw, E.e2, M);
 #pragma unroll
         for (int q = 0; q < 5; q++) F[q] += p0 * E.den * M[q];
-        mom_a<1, 0, 0>(fu, tv, tw, E.e2, E.e4, A, p1 * E.den, F);
-        const T cF = p2 * E.den;
-        mom_a<2, 0, 0>(hp, tv, tw, E.e2, E.e4, axl, cF, F);
-        mom_a<2, 0, 0>(hn, tv, tw, E.e2, E.e4, axr, cF, F);
-        mom_a<1, 1, 0>(fu, tv, tw, E.e2, E.e4, ay, cF, F);
-        mom_a<1, 0, 1>(fu, tv, tw, E.e2, E.e4, az, cF, F);
+        mom_a<1, 0, 0>(fu, tv, tw, E.e2, E.e4, a, p1 * E.den, F);
     }
 }
 
@@ -516,7 +529,7 @@ __host__ __device__ __forceinline__ void gks_flux_mixed(const double WL[5], cons
         for (int q = 0; q < 5; q++) Bz[q] = gm1 * E.den * M[q] + gm3 * We[q];
     }
 
-    // FP32: every slope-dependent term
+    // FP32: every slope-dependent term, state by state as in gks_flux
     const float fgm2 = (float)gm2, fgm45 = (float)(gm4 + gm5), fgm5 = (float)gm5;
     const float fp1 = (float)p1, fp2 = (float)p2, fp4 = (float)p4, fp5 = (float)p5;
     float Be[5], bye[5] = {0, 0, 0, 0, 0}, bze[5] = {0, 0, 0, 0, 0}, Ff[5] = {0, 0, 0, 0, 0};
@@ -526,12 +539,7 @@ __host__ __device__ __forceinline__ void gks_flux_mixed(const double WL[5], cons
     for (int side = 0; side < 2; side++) {
         const Maxw<float> S = to_f32(side == 0 ? L : R);
         const int dxi = side;
-        float hu[7], fu[6], tv[6], tw[6];
-        if (side == 0)
-            half_tab<7>(S.u, S.i2l, (float)hl0, (float)(L.u * hl0 + gl), hu);
-        else
-            half_tab<7>(S.u, S.i2l, (float)hr0, (float)(R.u * hr0 - gr), hu);
-        full_tab<6>(S.u, S.i2l, fu);
+        float tv[6], tw[6];
         full_tab<6>(S.v, S.i2l, tv);
         full_tab<6>(S.w, S.i2l, tw);
         const double iden = 1.0 / (side == 0 ? L.den : R.den);
@@ -545,14 +553,23 @@ __host__ __device__ __forceinline__ void gks_flux_mixed(const double WL[5], cons
 #pragma unroll
         for (int q = 0; q < 5; q++) b[q] = (float)(D[4 + dxi][q] * iden);
         slope(b, S, gf, az);
+        {
+            float fu[6];
+            full_tab<6>(S.u, S.i2l, fu);
+#pragma unroll
+            for (int q = 0; q < 5; q++) b[q] = 0.f;
+            mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, -1.f, b);
+            mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, -1.f, b);
+            mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, -1.f, b);
+            slope(b, S, gf, A);
+        }
+        float hu[7];
+        if (side == 0)
+            half_tab<7>(S.u, S.i2l, (float)hl0, (float)(L.u * hl0 + gl), hu);
+        else
+            half_tab<7>(S.u, S.i2l, (float)hr0, (float)(R.u * hr0 - gr), hu);
         mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, ay, 1.f, bye);
         mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, az, 1.f, bze);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = 0.f;
-        mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, -1.f, b);
-        mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, -1.f, b);
-        mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, -1.f, b);
-        slope(b, S, gf, A);
         const float cB = fgm45 * S.den;
         mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cB, Be);
         mom_a<0, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cB, Be);
@@ -568,38 +585,46 @@ __host__ __device__ __forceinline__ void gks_flux_mixed(const double WL[5], cons
         const Maxw<float> Ef = to_f32(E);
         const double ge = 0.5 * exp(-E.lam * E.u * E.u) / sqrt(E.lam * pi);
         const double sl = sqrt(E.lam) * E.u;
-        const double hp0 = 0.5 * erfc(-sl), hn0 = 0.5 * erfc(sl);
-        float hp[7], hn[7], fu[7], tv[6], tw[6];
-        half_tab<7>(Ef.u, Ef.i2l, (float)hp0, (float)(E.u * hp0 + ge), hp);
-        half_tab<7>(Ef.u, Ef.i2l, (float)hn0, (float)(E.u * hn0 - ge), hn);
-        full_tab<7>(Ef.u, Ef.i2l, fu);
+        float tv[6], tw[6];
         full_tab<6>(Ef.v, Ef.i2l, tv);
         full_tab<6>(Ef.w, Ef.i2l, tw);
         const double c = 2.0 / (E.den * dx);
-        float b[5], axl[5], axr[5], ay[5], az[5], A[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (float)((We[q] - WL[q]) * c);
-        slope(b, Ef, gf, axl);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (float)((WR[q] - We[q]) * c);
-        slope(b, Ef, gf, axr);
-        slope(bye, Ef, gf, ay);
-        slope(bze, Ef, gf, az);
         const float cB = fgm2 * Ef.den;
-        mom_a<1, 0, 0>(hp, tv, tw, Ef.e2, Ef.e4, axl, cB, Be);
-        mom_a<1, 0, 0>(hn, tv, tw, Ef.e2, Ef.e4, axr, cB, Be);
-        mom_a<0, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, ay, cB, Be);
-        mom_a<0, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, az, cB, Be);
+        const float cF = fp2 * Ef.den;
+        float b[5], a[5];
+        {
+            float h[7];
+            const double h0 = 0.5 * erfc(-sl);
+            half_tab<7>(Ef.u, Ef.i2l, (float)h0, (float)(E.u * h0 + ge), h);
+#pragma unroll
+            for (int q = 0; q < 5; q++) b[q] = (float)((We[q] - WL[q]) * c);
+            slope(b, Ef, gf, a);
+            mom_a<1, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
+            mom_a<2, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
+        }
+        {
+            float h[7];
+            const double h0 = 0.5 * erfc(sl);
+            half_tab<7>(Ef.u, Ef.i2l, (float)h0, (float)(E.u * h0 - ge), h);
+#pragma unroll
+            for (int q = 0; q < 5; q++) b[q] = (float)((WR[q] - We[q]) * c);
+            slope(b, Ef, gf, a);
+            mom_a<1, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
+            mom_a<2, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
+        }
+        float fu[7];
+        full_tab<7>(Ef.u, Ef.i2l, fu);
+        slope(bye, Ef, gf, a);
+        mom_a<0, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
+        mom_a<1, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
+        slope(bze, Ef, gf, a);
+        mom_a<0, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
+        mom_a<1, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
         const float iden = (float)(1.0 / E.den);
 #pragma unroll
         for (int q = 0; q < 5; q++) Be[q] *= iden;
-        slope(Be, Ef, gf, A);
-        mom_a<1, 0, 0>(fu, tv, tw, Ef.e2, Ef.e4, A, fp1 * Ef.den, Ff);
-        const float cF = fp2 * Ef.den;
-        mom_a<2, 0, 0>(hp, tv, tw, Ef.e2, Ef.e4, axl, cF, Ff);
-        mom_a<2, 0, 0>(hn, tv, tw, Ef.e2, Ef.e4, axr, cF, Ff);
-        mom_a<1, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, ay, cF, Ff);
-        mom_a<1, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, az, cF, Ff);
+        slope(Be, Ef, gf, a);
+        mom_a<1, 0, 0>(fu, tv, tw, Ef.e2, Ef.e4, a, fp1 * Ef.den, Ff);
     }
 #pragma unroll
     for (int q = 0; q < 5; q++) F[q] = F64[q] + (double)Ff[q];

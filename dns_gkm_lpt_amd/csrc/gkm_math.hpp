@@ -64,12 +64,13 @@ __host__ __device__ __forceinline__ T weno_side(T a, T b, T c, T d, T e, T f, T 
     T smax = s0 > s1 ? s0 : s1;
     smax = smax > s2 ? smax : s2;
     s3 = smax > s3 ? smax : s3;  // update.cpp:577-584
-    T a0 = T(0.05) / (eps + s0);
-    T a1 = T(0.45) / (eps + s1);
-    T a2 = T(0.45) / (eps + s2);
-    T a3 = T(0.05) / (eps + s3);
-    T inv = T(1) / (a0 + a1 + a2 + a3);
-    return (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) * inv;
+    // alpha_k = d_k/(eps+IS_k) normalised: with D_j = eps+IS_j the weights are
+    // d_k prod_{j!=k} D_j / sum_m d_m prod_{j!=m} D_j -- one division.
+    const T D0 = eps + s0, D1 = eps + s1, D2 = eps + s2, D3 = eps + s3;
+    const T p01 = D0 * D1, p23 = D2 * D3;
+    const T n0 = T(0.05) * D1 * p23, n1 = T(0.45) * D0 * p23;
+    const T n2 = T(0.45) * p01 * D3, n3 = T(0.05) * p01 * D2;
+    return (n0 * q0 + n1 * q1 + n2 * q2 + n3 * q3) / (n0 + n1 + n2 + n3);
 }
 
 // temperature of a conservative state (update.cpp:624-629)
@@ -134,8 +135,8 @@ __host__ __device__ __forceinline__ void maxw_init(Maxw<T>& s, const T W[5], con
     s.v = W[2] * ir;
     s.w = W[3] * ir;
     T P = (s.den * (g.gam - T(1))) * (W[4] * ir - T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w));
-    s.lam = T(0.5) * s.den / P;
-    s.i2l = P / s.den;  // 1/(2 lambda)
+    s.i2l = P * ir;  // 1/(2 lambda) = p/rho
+    s.lam = T(0.5) / s.i2l;
     s.e2 = T(g.K) * s.i2l;
     s.e4 = T(g.K * (g.K + 2)) * s.i2l * s.i2l;
 }
@@ -210,7 +211,8 @@ __host__ __device__ __forceinline__ void slope(const T b[5], const Maxw<T>& s, c
     T q = s.u * s.u + s.v * s.v + s.w * s.w + T(g.K + 3) * s.i2l;
     T R5 = T(2) * b[4] - b[0] * q;
     T lam2 = T(2) * s.lam;
-    a[4] = g.inv_prn * (R5 - T(2) * (s.u * R2 + s.v * R3 + s.w * R4)) * (lam2 * lam2) / T(g.K + 3);
+    // (4 lambda^2)/(K+3) = lam2^2 * 1/(K+3); K+3 is 5 for gamma = 1.4
+    a[4] = g.inv_prn * (R5 - T(2) * (s.u * R2 + s.v * R3 + s.w * R4)) * (lam2 * lam2) * (T(1) / T(g.K + 3));
     a[3] = lam2 * R4 - s.w * a[4];
     a[2] = lam2 * R3 - s.v * a[4];
     a[1] = lam2 * R2 - s.u * a[4];

@@ -505,22 +505,42 @@ int gkm_stats_compute(gkm_ctx* c, gkm_stats* out)
 
 int lpt_upload(gkm_ctx* c, long np, const double* pos)
 {
-    if (!c || np < 0 || (np > 0 && !pos)) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c || np < 0 || np > 0x7fffffffL || (np > 0 && !pos)) return fail(c, GKM_EINVAL, "bad argument");
     lpt::State& L = c->lpt;
-    hipFree(L.pos);
-    hipFree(L.U);
-    hipFree(L.G);
-    L.pos = L.U = L.G = nullptr;
+    const double twopi = 2.0 * (std::atan(1.0) * 4.0);
+    for (long p = 0; p < 3 * np; p++)
+        if (!(pos[p] >= 0.0 && pos[p] <= twopi)) return fail(c, GKM_EINVAL, "particle position outside [0, 2 pi]");
+    for (void* ptr : {(void*)L.pos, (void*)L.U, (void*)L.G, (void*)L.tag, (void*)L.tag2, (void*)L.pos2, (void*)L.keys,
+                      (void*)L.perm, L.sort_tmp})
+        (void)hipFree(ptr);
+    L.pos = L.U = L.G = L.pos2 = nullptr;
+    L.tag = L.tag2 = L.perm = nullptr;
+    L.keys = nullptr;
+    L.sort_tmp = nullptr;
+    L.sort_tmp_bytes = 0;
     L.np = np;
     L.sampled = false;
     if (np == 0) return GKM_OK;
     if (hipMalloc(&L.pos, sizeof(double) * 3 * np) != hipSuccess || hipMalloc(&L.U, sizeof(double) * 3 * np) != hipSuccess ||
-        hipMalloc(&L.G, sizeof(double) * 9 * np) != hipSuccess)
+        hipMalloc(&L.G, sizeof(double) * 9 * np) != hipSuccess || hipMalloc(&L.pos2, sizeof(double) * 9 * np) != hipSuccess ||
+        hipMalloc(&L.tag, sizeof(int) * np) != hipSuccess || hipMalloc(&L.tag2, sizeof(int) * np) != hipSuccess ||
+        hipMalloc(&L.keys, sizeof(unsigned) * 2 * np) != hipSuccess || hipMalloc(&L.perm, sizeof(int) * 2 * np) != hipSuccess)
         return fail(c, GKM_ENOMEM, "hipMalloc particles");
-    const double twopi = 2.0 * (std::atan(1.0) * 4.0);
-    for (long p = 0; p < 3 * np; p++)
-        if (!(pos[p] >= 0.0 && pos[p] <= twopi)) return fail(c, GKM_EINVAL, "particle position outside [0, 2 pi]");
+    std::vector<int> tags(np);
+    for (long p = 0; p < np; p++) tags[p] = (int)p;
     HIPCHK(c, hipMemcpyAsync(L.pos, pos, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(L.tag, tags.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, lpt::sort_by_cell(L, geom_of(c), c->dx, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+// copy a per-particle array (slot order, `width` doubles per particle) to the host in tag order
+static int download_tag_order(gkm_ctx* c, const double* src, int width, double* host)
+{
+    lpt::State& L = c->lpt;
+    HIPCHK(c, lpt::scatter_to_tags(L.np, L.tag, src, width, L.pos2, c->stream));
+    HIPCHK(c, hipMemcpyAsync(host, L.pos2, sizeof(double) * width * L.np, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }
@@ -530,10 +550,10 @@ int lpt_download(gkm_ctx* c, double* pos, double* vel)
     if (!c) return GKM_EINVAL;
     const lpt::State& L = c->lpt;
     if (L.np == 0) return GKM_OK;
-    if (pos) HIPCHK(c, hipMemcpyAsync(pos, L.pos, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
-    if (vel) HIPCHK(c, hipMemcpyAsync(vel, L.U, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return GKM_OK;
+    int rc = GKM_OK;
+    if (pos) rc = download_tag_order(c, L.pos, 3, pos);
+    if (!rc && vel) rc = download_tag_order(c, L.U, 3, vel);
+    return rc;
 }
 
 static int sample_device(gkm_ctx* c)
@@ -557,18 +577,20 @@ int lpt_sample(gkm_ctx* c, double* U, double* gU, double* gV, double* gW)
     if (L.np == 0) return GKM_OK;
     int rc = sample_device(c);
     if (rc) return rc;
-    if (U) HIPCHK(c, hipMemcpyAsync(U, L.U, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
+    if (U) {
+        rc = download_tag_order(c, L.U, 3, U);
+        if (rc) return rc;
+    }
     if (gU || gV || gW) {
         std::vector<double> G((size_t)9 * L.np);
-        HIPCHK(c, hipMemcpyAsync(G.data(), L.G, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        rc = download_tag_order(c, L.G, 9, G.data());
+        if (rc) return rc;
         double* outs[3] = {gU, gV, gW};
         for (int f = 0; f < 3; f++)
             if (outs[f])
                 for (long p = 0; p < L.np; p++)
                     for (int a = 0; a < 3; a++) outs[f][3 * p + a] = G[(size_t)9 * p + 3 * f + a];
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }
 
@@ -606,6 +628,7 @@ int lpt_heun(gkm_ctx* c, double dt)
     const double* cn[3] = {L.coef[L.cur][0], L.coef[L.cur][1], L.coef[L.cur][2]};
     rc = timed(c, 5, [&] { return lpt::launch_heun(L.np, L.pos, L.U, cn, g, c->dscal + 2, c->stream); });
     if (rc) return rc;
+    if (++L.since_sort >= L.sort_every) HIPCHK(c, lpt::sort_by_cell(L, g, c->dx, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }
@@ -629,6 +652,7 @@ int lpt_coupled_step(gkm_ctx* c, double dt)
     const double* cn[3] = {L.coef[nb][0], L.coef[nb][1], L.coef[nb][2]};
     rc = timed(c, 5, [&] { return lpt::launch_heun(L.np, L.pos, L.U, cn, g, c->dscal, c->stream); });
     if (rc) return rc;
+    if (++L.since_sort >= L.sort_every) HIPCHK(c, lpt::sort_by_cell(L, g, c->dx, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }

@@ -24,6 +24,12 @@ void release(State& s)
         }
     (void)hipFree(s.pivots);
     (void)hipFree(s.red);
+    (void)hipFree(s.tag);
+    (void)hipFree(s.tag2);
+    (void)hipFree(s.pos2);
+    (void)hipFree(s.keys);
+    (void)hipFree(s.perm);
+    (void)hipFree(s.sort_tmp);
     s = State();
 }
 

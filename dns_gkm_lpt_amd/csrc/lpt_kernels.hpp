@@ -17,6 +17,17 @@ struct State {
     double* pivots = nullptr;  // Thomas pivots 1/(4 - p_{r-1}) of the [1 4 1] system
     double* red = nullptr;     // reduction scratch
     bool sampled = false;
+    // cell sort: particles are kept ordered by the cell holding them, tag[i]
+    // is the upload index of the particle stored in slot i
+    int* tag = nullptr;
+    int* tag2 = nullptr;
+    double* pos2 = nullptr;    // np x 9 scratch (gathers, tag-order outputs)
+    unsigned* keys = nullptr;  // 2 x np
+    int* perm = nullptr;       // 2 x np
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int sort_every = 10;       // re-sort after this many Heun steps
+    int since_sort = 0;
 };
 
 void release(State& s);
@@ -36,6 +47,12 @@ hipError_t launch_sample(long np, const double* pos, const double* const cu[3], 
                          hipStream_t s);
 hipError_t launch_heun(long np, double* pos, const double* U, const double* const cn[3], const Geom& g,
                        const double* dt, hipStream_t s);
+
+// Reorder the particles by cell index (y, x, z cell of dx) with a radix sort;
+// pos and tag follow, U and G are resampled after a sort.
+hipError_t sort_by_cell(State& s, const Geom& g, double dx, hipStream_t st);
+// dst[tag[i]] = src[i] for rows of `width` doubles (slot order -> tag order)
+hipError_t scatter_to_tags(long np, const int* tag, const double* src, int width, double* dst, hipStream_t st);
 
 // flow statistics (PrintStats, update.cpp:688-842); red holds >= kStatBlocks*32 doubles
 constexpr int kStatBlocks = 512;

@@ -38,14 +38,15 @@ def log(msg):
 
 def cpu_baseline(seconds_cap=30.0):
     """Time the CPU Eulerian step on a bounded sample of the same workload
-    (one step of a 64^3 cube of the same initial-condition recipe)."""
+    (one step of a 128^3 cube of the same initial-condition recipe, ~20 s on
+    one core)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes
 
     import pyoracle as po
     from dns_gkm_lpt_amd import ic as icm
 
-    nc = 64
+    nc = 128
     ic = icm.make_ic(nc, k0=4.0, Mt=0.5, Re=100.0, seed=0)
     P = ic["params"]
     W = icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"])
@@ -70,6 +71,22 @@ def cpu_baseline(seconds_cap=30.0):
             "sample": f"1 step of {nc}^3 (Mt 0.5, Re_lambda 100, k0 4, seed 0), {what}, {el:.2f} s"}
 
 
+def copy_calibration(mib):
+    """One plain device copy of `mib` MiB (a fill, then an elementwise negation
+    out of place) so that FETCH_SIZE / WRITE_SIZE of a counter run can be
+    scaled by what they read for a kernel of known traffic."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.full((n,), 2.0, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"bench: copy calibration {mib} MiB read + {mib} MiB written (torch.neg), fill {mib} MiB written")
+    del a, b
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +96,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain copy of this size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the periodic cube is resident on a single MI355X and this build has no "
@@ -88,6 +107,10 @@ def main():
     from dns_gkm_lpt_amd import _lib as gl
     from dns_gkm_lpt_amd import ic as icm
 
+    if args.profile_steps < 1:
+        ap.error("--profile-steps must be >= 1")
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
     nc = args.nc
     t0 = time.perf_counter()
     ic = icm.make_ic(nc, k0=4.0, Mt=0.5, Re=100.0, seed=0)

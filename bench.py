@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
+    ap.add_argument("--tiles", default="auto",
+                    help="sweep tile variants x,y,z (e.g. 0,0,0) or 'auto' to time them before the run")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -123,6 +125,12 @@ def main():
     ctx.upload(W)
     del W
     log("bench: state uploaded")
+    if args.tiles == "auto":
+        tiles, tms = ctx.autotune(reps=2)
+        log(f"bench: autotuned sweep tiles {tiles}; ms per variant (rows x, y, z) {np.round(tms, 2).tolist()}")
+    else:
+        ctx.set_tiles([int(v) for v in args.tiles.split(",")])
+        tiles = ctx.tiles()
     ctx.run(args.warmup)
     ctx.sync()
     log(f"bench: {args.warmup} warm-up steps done")
@@ -165,7 +173,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"GKM-WENO Eulerian step (CFL 0.5 dt on device + fused x/y/z sweeps + update), "
                                f"{nc}^3 periodic cube, Mt 0.5, Re_lambda 100, k0 4, seed 0",
-                   "nc": nc, "precision": args.precision},
+                   "nc": nc, "precision": args.precision, "sweep_tiles": list(tiles)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": names[dom], "avg_ms": dom_ms,

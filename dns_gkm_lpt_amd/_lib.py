@@ -19,7 +19,7 @@ ERRORS = {1: "GKM_EINVAL", 2: "GKM_EHIP", 3: "GKM_ENOMEM", 4: "GKM_ESTATE", 5: "
 EXPORTS = [
     "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
-    "gkm_sweep_flux_diff",
+    "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
     "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_coupled_step", "lpt_heun",
 ]
@@ -75,6 +75,9 @@ def lib():
         L.gkm_run.argtypes = [P, C.c_int, C.c_void_p]
         L.gkm_sync.argtypes = [P]
         L.gkm_sweep_flux_diff.argtypes = [P, C.c_int, C.c_double, _dp]
+        L.gkm_set_sweep_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
+        L.gkm_get_sweep_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
+        L.gkm_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int * 3), C.POINTER(C.c_double * 9)]
         L.gkm_stats_compute.argtypes = [P, C.POINTER(Stats)]
         L.gkm_timing_enable.argtypes = [P, C.c_int]
         L.gkm_timing_read.argtypes = [P, C.POINTER(C.c_double * 8), C.POINTER(C.c_long * 8)]
@@ -169,6 +172,23 @@ class Context:
 
     def sync(self):
         self._chk(lib().gkm_sync(self._h))
+
+    def set_tiles(self, tiles):
+        t = (C.c_int * 3)(*tiles)
+        self._chk(lib().gkm_set_sweep_tiles(self._h, C.byref(t)))
+
+    def tiles(self):
+        t = (C.c_int * 3)()
+        self._chk(lib().gkm_get_sweep_tiles(self._h, C.byref(t)))
+        return tuple(t)
+
+    def autotune(self, reps=2):
+        """Time every sweep tile variant on the resident state, keep the fastest;
+        returns (tiles, ms) with ms[d][v] the average launch time."""
+        t = (C.c_int * 3)()
+        ms = (C.c_double * 9)()
+        self._chk(lib().gkm_autotune(self._h, reps, C.byref(t), C.byref(ms)))
+        return tuple(t), np.array(ms[:]).reshape(3, 3)
 
     def stats(self):
         s = Stats()

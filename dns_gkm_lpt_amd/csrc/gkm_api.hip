@@ -34,6 +34,7 @@ struct gkm_ctx {
     int* dflag = nullptr;          // non-physical-state flag
     double* partial = nullptr;     // CFL block maxima
     bool have_state = false;
+    int tile[3] = {0, 0, 0};       // sweep tile variant per direction
     std::string err;
     // per-kernel timing
     bool timing = false;
@@ -80,7 +81,7 @@ int timed(gkm_ctx* c, int slot, Fn f)
     return GKM_OK;
 }
 
-int eulerian_step(gkm_ctx* c)
+gkm::SweepArgs sweep_args(const gkm_ctx* c, int dir, int mode)
 {
     gkm::SweepArgs a{};
     a.W = c->W[c->cur];
@@ -92,10 +93,18 @@ int eulerian_step(gkm_ctx* c)
     a.dx = c->dx;
     a.inv_dx = 1.0 / c->dx;
     a.g = c->gas;
+    a.tile = c->tile[dir];
+    a.seg = gkm::sweep_seg_len(dir, c->nc, a.tile);
+    a.nseg = c->nc / a.seg;
+    a.mode = mode;
+    return a;
+}
+
+int eulerian_step(gkm_ctx* c)
+{
     for (int dir = 0; dir < 3; dir++) {
-        a.seg = gkm::sweep_seg_len(dir, c->nc);
-        a.nseg = c->nc / a.seg;
-        a.mode = dir;  // x: dW = dF, y: dW += dF, z: Wout = W - (dW + dF)/dx
+        // x: dW = dF, y: dW += dF, z: Wout = W - (dW + dF)/dx
+        const gkm::SweepArgs a = sweep_args(c, dir, dir);
         int rc = timed(c, dir, [&] { return gkm::launch_sweep(dir, a, c->cfg.precision == GKM_MIXED, c->stream); });
         if (rc) return rc;
     }
@@ -282,23 +291,69 @@ int gkm_sweep_flux_diff(gkm_ctx* c, int dir, double dt, double* out)
     if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, GKM_EINVAL, "dt must be positive and finite");
     HIPCHK(c, hipMemcpyAsync(c->dscal, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    gkm::SweepArgs a{};
-    a.W = c->W[c->cur];
-    a.dW = c->dW;
-    a.Wout = c->W[1 - c->cur];
-    a.dt = c->dscal;
-    a.plane = c->plane;
-    a.nc = c->nc;
-    a.dx = c->dx;
-    a.inv_dx = 1.0 / c->dx;
-    a.g = c->gas;
-    a.seg = gkm::sweep_seg_len(dir, c->nc);
-    a.nseg = c->nc / a.seg;
-    a.mode = 0;
+    const gkm::SweepArgs a = sweep_args(c, dir, 0);
     int rc = timed(c, dir, [&] { return gkm::launch_sweep(dir, a, c->cfg.precision == GKM_MIXED, c->stream); });
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(out, c->dW, sizeof(double) * 5 * c->plane, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int gkm_set_sweep_tiles(gkm_ctx* c, const int tile[3])
+{
+    if (!c || !tile) return fail(c, GKM_EINVAL, "null argument");
+    for (int d = 0; d < 3; d++)
+        if (tile[d] < 0 || tile[d] >= gkm::kTileVariants) return fail(c, GKM_EINVAL, "tile variant out of range");
+    for (int d = 0; d < 3; d++) c->tile[d] = tile[d];
+    return GKM_OK;
+}
+
+int gkm_get_sweep_tiles(const gkm_ctx* c, int tile[3])
+{
+    if (!c || !tile) return GKM_EINVAL;
+    for (int d = 0; d < 3; d++) tile[d] = c->tile[d];
+    return GKM_OK;
+}
+
+int gkm_autotune(gkm_ctx* c, int reps, int tile[3], double ms[9])
+{
+    if (!c || reps < 1) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    // the CFL dt of the resident state, so each candidate does the step's real work
+    int rc = cfl_on_device(c);
+    if (rc) return rc;
+    const bool mixed = c->cfg.precision == GKM_MIXED;
+    for (int dir = 0; dir < 3; dir++) {
+        double best = 0.0;
+        int best_v = c->tile[dir];
+        for (int v = 0; v < gkm::kTileVariants; v++) {
+            gkm::SweepArgs a = sweep_args(c, dir, dir);
+            a.tile = v;
+            a.seg = gkm::sweep_seg_len(dir, c->nc, v);
+            a.nseg = c->nc / a.seg;
+            // dW is scratch and Wout is the next state's buffer: the resident state is not changed
+            hipError_t e = gkm::launch_sweep(dir, a, mixed, c->stream);  // warm-up
+            if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps; r++) {
+                e = gkm::launch_sweep(dir, a, mixed, c->stream);
+                if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+            }
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev[1]));
+            float el = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&el, c->ev[0], c->ev[1]));
+            const double avg = el / reps;
+            if (ms) ms[3 * dir + v] = avg;
+            if (v == 0 || avg < best) {
+                best = avg;
+                best_v = v;
+            }
+        }
+        c->tile[dir] = best_v;
+    }
+    if (tile)
+        for (int d = 0; d < 3; d++) tile[d] = c->tile[d];
     return GKM_OK;
 }
 

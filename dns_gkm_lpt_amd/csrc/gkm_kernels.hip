@@ -53,22 +53,71 @@ struct Lines {
     }
 };
 
-// MIX = false: the FP64 path.  MIX = true: WENO in FP32 on FP64 stencil
-// differences and the mixed-precision flux (gks_flux_mixed).
-// Two waves per SIMD are requested: the register budget is then 256 per lane
-// and the flux's overflow goes to scratch.  With one wave per SIMD the
-// compiler (ROCm 7.2, gfx950) spills into AGPRs and the x and y instances
-// produced wrong flux differences on the GPU (the z instance and every
-// instance built this way match the FP64 oracle to 1e-14).
-template <int DIR, bool MIX>
-__global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) void sweep_kernel(SweepArgs p)
+// LDS layout of the WENO states (10 values per face and line) and of the face
+// fluxes (5 values per face slot and interior line).  The fastest thread index
+// is contiguous: the tangential line for the x and y sweeps, the face for the z
+// sweep (whose lines are short and whose faces are consecutive in memory).
+template <int DIR, int CN, int L>
+__device__ __forceinline__ int sw_at(int v, int tn, int l)
 {
-    constexpr int CN = Tile<DIR>::CN, CA = Tile<DIR>::CA, CB = Tile<DIR>::CB;
+    return DIR == 2 ? (v * L + l) * CN + tn : (v * CN + tn) * L + l;
+}
+template <int DIR, int CN, int NL>
+__device__ __forceinline__ int sf_at(int q, int slot, int lc)
+{
+    return DIR == 2 ? (q * NL + lc) * (CN + 1) + slot : (q * (CN + 1) + slot) * NL + lc;
+}
+
+// Flux inputs of one face, read where the flux uses them: cell states from the
+// (cached) state array, WENO states and their tangential differences from LDS.
+// q is the rotated-frame slot (normal momentum in slot 1; update.cpp:361-398, 461-497).
+template <int DIR, int CN, int L>
+struct SweepIn {
+    const double* __restrict__ W;
+    long plane, iL, iR;
+    const double* s;
+    int tn, lc, l_am, l_ap, l_bm, l_bp;
+    double two_idx, half_idx;
+    __device__ static int phys(int q)
+    {
+        constexpr int sw = DIR == 1 ? 2 : 3;
+        return DIR == 0 ? q : (q == 1 ? sw : (q == sw ? 1 : q));
+    }
+    __device__ double at(int v, int l) const { return s[sw_at<DIR, CN, L>(v, tn, l)]; }
+    __device__ double WL(int q) const { return W[phys(q) * plane + iL]; }
+    __device__ double WR(int q) const { return W[phys(q) * plane + iR]; }
+    __device__ double Wl(int q) const { return at(phys(q), lc); }
+    __device__ double Wr(int q) const { return at(5 + phys(q), lc); }
+    __device__ double D(int d, int q) const
+    {
+        const int p = phys(q);
+        switch (d) {
+        case 0: return (at(p, lc) - WL(q)) * two_idx;                 // DWxl
+        case 1: return (WR(q) - at(5 + p, lc)) * two_idx;             // DWxr
+        case 2: return (at(p, l_ap) - at(p, l_am)) * half_idx;        // DWyl
+        case 3: return (at(5 + p, l_ap) - at(5 + p, l_am)) * half_idx;  // DWyr
+        case 4: return (at(p, l_bp) - at(p, l_bm)) * half_idx;        // DWzl
+        default: return (at(5 + p, l_bp) - at(5 + p, l_bm)) * half_idx;  // DWzr
+        }
+    }
+};
+
+// MIX = false: the FP64 path.  MIX = true: WENO in FP32 on FP64 stencil
+// differences and the flux with FP32 slope terms (gks_flux_tf<double, float>).
+// Two waves per SIMD are requested: the register budget is then 256 per lane.
+// With one wave per SIMD the compiler (ROCm 7.2, gfx950) spilled into AGPRs and
+// the x and y instances produced wrong flux differences on the GPU (the z
+// instance and every instance built this way match the FP64 oracle to 1e-14).
+template <int DIR, bool MIX, int V>
+__global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
+{
+    constexpr int CN = Tile<DIR, V>::CN, CA = Tile<DIR, V>::CA, CB = Tile<DIR, V>::CB;
     constexpr int NT = CN * CA * CB;
+    static_assert(NT == 256, "every tile variant runs 256 threads, one face each");
     using LN = Lines<CA, CB>;
     constexpr int L = LN::L;
-    __shared__ double sW[10][CN][L];          // WENO states: Wl (0..4), Wr (5..9)
-    __shared__ double sF[5][CN + 1][CA * CB]; // face fluxes; slot 0 = face before the chunk
+    __shared__ double sW[10 * CN * L];          // WENO states: Wl (0..4), Wr (5..9)
+    __shared__ double sF[5 * (CN + 1) * CA * CB]; // face fluxes; slot 0 = face before the chunk
 
     const int nc = p.nc;
     const long plane = p.plane;
@@ -89,22 +138,32 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) voi
     int tn, ta, tb;
     if (DIR == 2) {
         tn = tid % CN;
-        ta = (tid / CN) % CA;
-        tb = tid / (CN * CA);
+        tb = (tid / CN) % CB;
+        ta = tid / (CN * CB);
     } else {
         tb = tid % CB;
         ta = (tid / CB) % CA;
         tn = tid / (CA * CB);
     }
     const int lc = ta * CB + tb;
-    const int l_am = LN::id(ta - 1, tb), l_ap = LN::id(ta + 1, tb);
-    const int l_bm = LN::id(ta, tb - 1), l_bp = LN::id(ta, tb + 1);
 
     const double dt = *p.dt;
     const double dx = p.dx;
-    const double two_idx = 2.0 * p.inv_dx, half_idx = 0.5 * p.inv_dx;
     const int nchunk = p.seg / CN;
     const int seg0 = s * p.seg;
+
+    SweepIn<DIR, CN, L> in;
+    in.W = W;
+    in.plane = plane;
+    in.s = sW;
+    in.tn = tn;
+    in.lc = lc;
+    in.l_am = LN::id(ta - 1, tb);
+    in.l_ap = LN::id(ta + 1, tb);
+    in.l_bm = LN::id(ta, tb - 1);
+    in.l_bp = LN::id(ta, tb + 1);
+    in.two_idx = 2.0 * p.inv_dx;
+    in.half_idx = 0.5 * p.inv_dx;
 
     for (int c = -1; c < nchunk; ++c) {
         const int n0 = seg0 + c * CN;  // first face of the chunk (may be < 0 for the lead-in)
@@ -132,8 +191,8 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) voi
                 weno5<double>(S, p.g, Wl, Wr);
 #pragma unroll
             for (int q = 0; q < 5; q++) {
-                sW[q][itn][il] = Wl[q];
-                sW[5 + q][itn][il] = Wr[q];
+                sW[sw_at<DIR, CN, L>(q, itn, il)] = Wl[q];
+                sW[sw_at<DIR, CN, L>(5 + q, itn, il)] = Wr[q];
             }
         }
         __syncthreads();
@@ -142,45 +201,22 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) voi
         {
             const int fn = n0 + tn;
             const int ca = a0 + ta, cb = b0 + tb;  // interior lines: in range
-            const long iL = cell_index<DIR>(wrapi(fn, nc), ca, cb, nc);
-            const long iR = cell_index<DIR>(wrapi(fn + 1, nc), ca, cb, nc);
-            double WL[5], WR[5], Wl[5], Wr[5], D[6][5];
+            in.iL = cell_index<DIR>(wrapi(fn, nc), ca, cb, nc);
+            in.iR = cell_index<DIR>(wrapi(fn + 1, nc), ca, cb, nc);
+            double tau;
+            {
+                double WL[5];
 #pragma unroll
-            for (int q = 0; q < 5; q++) {
-                WL[q] = W[q * plane + iL];
-                WR[q] = W[q * plane + iR];
-                Wl[q] = sW[q][tn][lc];
-                Wr[q] = sW[5 + q][tn][lc];
-                D[0][q] = (Wl[q] - WL[q]) * two_idx;
-                D[1][q] = (WR[q] - Wr[q]) * two_idx;
-                D[2][q] = (sW[q][tn][l_ap] - sW[q][tn][l_am]) * half_idx;
-                D[3][q] = (sW[5 + q][tn][l_ap] - sW[5 + q][tn][l_am]) * half_idx;
-                D[4][q] = (sW[q][tn][l_bp] - sW[q][tn][l_bm]) * half_idx;
-                D[5][q] = (sW[5 + q][tn][l_bp] - sW[5 + q][tn][l_bm]) * half_idx;
-            }
-            const double tau = collision_time<double>(WL, p.g);
-            if (DIR != 0) {  // rotated frame: normal momentum into slot 1 (update.cpp:361-398, 461-497)
-                constexpr int sw = DIR == 1 ? 2 : 3;
-                double t;
-#define GKM_SWAP(v) t = v[sw]; v[sw] = v[1]; v[1] = t;
-                GKM_SWAP(WL) GKM_SWAP(WR) GKM_SWAP(Wl) GKM_SWAP(Wr)
-#pragma unroll
-                for (int d = 0; d < 6; d++) { GKM_SWAP(D[d]) }
-#undef GKM_SWAP
+                for (int q = 0; q < 5; q++) WL[q] = W[q * plane + in.iL];
+                tau = collision_time<double>(WL, p.g);  // symmetric in the momentum slots
             }
             double F[5];
             if (MIX)
-                gks_flux_mixed(WL, WR, Wl, Wr, D, tau, dt, dx, p.g, F);
+                gks_flux_tf<double, float>(in, tau, dt, dx, p.g, F);
             else
-                gks_flux<double>(WL, WR, Wl, Wr, D, tau, dt, dx, p.g, F);
-            if (DIR != 0) {
-                constexpr int sw = DIR == 1 ? 2 : 3;
-                double t = F[sw];
-                F[sw] = F[1];
-                F[1] = t;
-            }
+                gks_flux_tf<double, double>(in, tau, dt, dx, p.g, F);
 #pragma unroll
-            for (int q = 0; q < 5; q++) sF[q][tn + 1][lc] = F[q];
+            for (int q = 0; q < 5; q++) sF[sf_at<DIR, CN, CA * CB>(in.phys(q), tn + 1, lc)] = F[q];
         }
         __syncthreads();
 
@@ -189,7 +225,7 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) voi
             const long idx = cell_index<DIR>(n0 + tn, a0 + ta, b0 + tb, nc);
 #pragma unroll
             for (int q = 0; q < 5; q++) {
-                const double dF = sF[q][tn + 1][lc] - sF[q][tn][lc];
+                const double dF = sF[sf_at<DIR, CN, CA * CB>(q, tn + 1, lc)] - sF[sf_at<DIR, CN, CA * CB>(q, tn, lc)];
                 const long o = q * plane + idx;
                 if (p.mode == 0) p.dW[o] = dF;
                 else if (p.mode == 1) p.dW[o] += dF;
@@ -199,7 +235,7 @@ __global__ __launch_bounds__(Tile<DIR>::CN* Tile<DIR>::CA* Tile<DIR>::CB, 2) voi
         __syncthreads();
         if (tn == CN - 1) {
 #pragma unroll
-            for (int q = 0; q < 5; q++) sF[q][0][lc] = sF[q][CN][lc];
+            for (int q = 0; q < 5; q++) sF[sf_at<DIR, CN, CA * CB>(q, 0, lc)] = sF[sf_at<DIR, CN, CA * CB>(q, CN, lc)];
         }
         // the next write of sF slots 1..CN happens after the next chunk's first barrier
     }
@@ -213,38 +249,51 @@ bool sweep_grid_ok(int nc)
     return nc >= 32 && nc % 16 == 0;
 }
 
-int sweep_seg_len(int dir, int nc)
+int sweep_seg_len(int dir, int nc, int tile)
 {
     // faces per workgroup: the whole line when that already gives >= 1024
     // workgroups (4 per CU), otherwise split; each split costs one lead-in
-    // chunk, so segments stay >= 8 chunks long.
-    const int CA = dir == 2 ? Tile<2>::CA : Tile<0>::CA;
-    const int CB = dir == 2 ? Tile<2>::CB : Tile<0>::CB;
-    const int CN = dir == 2 ? Tile<2>::CN : Tile<0>::CN;
+    // chunk, so segments stay >= 8 chunks and >= 64 faces long.
+    if (dir < 0 || dir > 2 || tile < 0 || tile >= kTileVariants) return 0;
+    const int CN = kTiles[dir][tile][0], CA = kTiles[dir][tile][1], CB = kTiles[dir][tile][2];
     const long pencils = (long)(nc / CA) * (nc / CB);
+    const int min_seg = 8 * CN > 64 ? 8 * CN : 64;
     int seg = nc;
-    while (pencils * (nc / seg) < 1024 && seg % 2 == 0 && (seg / 2) % CN == 0 && seg / 2 >= 8 * CN) seg /= 2;
+    while (pencils * (nc / seg) < 1024 && seg % 2 == 0 && (seg / 2) % CN == 0 && seg / 2 >= min_seg) seg /= 2;
     return seg;
+}
+
+template <int DIR, int V>
+static hipError_t launch_tile(const SweepArgs& a, bool mixed, hipStream_t s)
+{
+    constexpr int CN = Tile<DIR, V>::CN, CA = Tile<DIR, V>::CA, CB = Tile<DIR, V>::CB;
+    if (a.nc % CA || a.nc % CB || a.seg % CN || a.nc % a.seg || a.nseg != a.nc / a.seg) return hipErrorInvalidValue;
+    const long nblk = (long)(a.nc / CA) * (a.nc / CB) * a.nseg;
+    if (mixed)
+        hipLaunchKernelGGL((sweep_kernel<DIR, true, V>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
+    else
+        hipLaunchKernelGGL((sweep_kernel<DIR, false, V>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
+    return hipGetLastError();
 }
 
 template <int DIR>
 static hipError_t launch_dir(const SweepArgs& a, bool mixed, hipStream_t s)
 {
-    constexpr int CN = Tile<DIR>::CN, CA = Tile<DIR>::CA, CB = Tile<DIR>::CB;
-    if (a.nc % CA || a.nc % CB || a.seg % CN || a.nc % a.seg || a.nseg != a.nc / a.seg) return hipErrorInvalidValue;
-    const long nblk = (long)(a.nc / CA) * (a.nc / CB) * a.nseg;
-    if (mixed)
-        hipLaunchKernelGGL((sweep_kernel<DIR, true>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
-    else
-        hipLaunchKernelGGL((sweep_kernel<DIR, false>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
-    return hipGetLastError();
+    static_assert(kTileVariants == 3, "one case per tile variant");
+    switch (a.tile) {
+    case 0: return launch_tile<DIR, 0>(a, mixed, s);
+    case 1: return launch_tile<DIR, 1>(a, mixed, s);
+    case 2: return launch_tile<DIR, 2>(a, mixed, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s)
 {
     if (dir == 0) return launch_dir<0>(a, mixed, s);
     if (dir == 1) return launch_dir<1>(a, mixed, s);
-    return launch_dir<2>(a, mixed, s);
+    if (dir == 2) return launch_dir<2>(a, mixed, s);
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -402,7 +451,7 @@ __global__ __launch_bounds__(64) void flux_batch_kernel(long n, const double* __
 #pragma unroll
         for (int d = 0; d < 6; d++) D[d][q] = (T)v[20 + 5 * d + q];
     }
-    gks_flux<T>(WL, WR, Wl, Wr, D, (T)tau[i], (T)dt, (T)dx, g, Fo);
+    gks_flux_tf<T, T>(WL, WR, Wl, Wr, D, (T)tau[i], (T)dt, (T)dx, g, Fo);
 #pragma unroll
     for (int q = 0; q < 5; q++) F[5 * i + q] = (double)Fo[q];
 }
@@ -424,7 +473,7 @@ __global__ __launch_bounds__(64) void flux_batch_mixed_kernel(long n, const doub
 #pragma unroll
         for (int d = 0; d < 6; d++) D[d][q] = v[20 + 5 * d + q];
     }
-    gks_flux_mixed(WL, WR, Wl, Wr, D, tau[i], dt, dx, g, Fo);
+    gks_flux_tf<double, float>(WL, WR, Wl, Wr, D, tau[i], dt, dx, g, Fo);
 #pragma unroll
     for (int q = 0; q < 5; q++) F[5 * i + q] = Fo[q];
 }

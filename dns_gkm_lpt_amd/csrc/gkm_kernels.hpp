@@ -1,7 +1,7 @@
 // gkm_kernels.hpp -- launch interface of the gfx950 kernels of the Eulerian step.
 #pragma once
 #include <hip/hip_runtime.h>
-#include "gkm_math.hpp"
+#include "gkm_flux.hpp"
 
 namespace gkm {
 
@@ -18,19 +18,30 @@ struct SweepArgs {
     int seg;             // faces per workgroup along the sweep axis
     int nseg;            // nc / seg
     int mode;            // 0: dW = dF   1: dW += dF   2: Wout = W - (dW + dF)/dx
+    int tile;            // tile shape variant, 0 .. kTileVariants-1
     double dx, inv_dx;
     Gas<double> g;
 };
 
-// Sweep tile shapes (faces along the sweep axis x two tangential axes).
-template <int DIR> struct Tile;
-template <> struct Tile<0> { static constexpr int CN = 4, CA = 4, CB = 16; };   // x sweep: a = y, b = z
-template <> struct Tile<1> { static constexpr int CN = 4, CA = 4, CB = 16; };   // y sweep: a = x, b = z
-template <> struct Tile<2> { static constexpr int CN = 16, CA = 4, CB = 4; };   // z sweep: a = x, b = y
+// Sweep tile shapes: CN faces along the sweep axis x CA x CB tangential lines
+// (256 threads, one face each).  A chunk reconstructs CN x (CA CB + 2 CA + 2 CB)
+// WENO states for CN x CA x CB fluxes; the tangential edge lines are the price
+// of the y/z slopes.  x sweep: a = y, b = z; y sweep: a = x, b = z; z sweep:
+// a = x, b = y.
+constexpr int kTileVariants = 3;
+constexpr int kTiles[3][kTileVariants][3] = {
+    {{4, 4, 16}, {1, 16, 16}, {2, 8, 16}},
+    {{4, 4, 16}, {1, 16, 16}, {2, 8, 16}},
+    {{16, 4, 4}, {4, 8, 8}, {1, 16, 16}},
+};
+template <int DIR, int V>
+struct Tile {
+    static constexpr int CN = kTiles[DIR][V][0], CA = kTiles[DIR][V][1], CB = kTiles[DIR][V][2];
+};
 
 hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s);
-int sweep_seg_len(int dir, int nc);  // faces per workgroup chosen for a grid
-bool sweep_grid_ok(int nc);          // nc compatible with every tile shape
+int sweep_seg_len(int dir, int nc, int tile);  // faces per workgroup chosen for a grid
+bool sweep_grid_ok(int nc);                    // nc compatible with every tile shape
 
 // CFL: partial maxima (one per block) then a one-block final pass writing
 // dt = cfl*dx/max and a non-physical-state flag.

@@ -98,6 +98,18 @@ int gkm_run(gkm_ctx *ctx, int nsteps, double *dt_hist);
  * terms update() sums (update.cpp:505-516).  The state is not changed.
  * A per-direction parity hook for derivsX/Y/Z (update.cpp:214-503). */
 int gkm_sweep_flux_diff(gkm_ctx *ctx, int dir, double dt, double *out);
+/* Sweep tile shape per direction (x, y, z): variant 0..2 of
+ * {faces along the sweep axis} x {tangential lines} per workgroup; results do
+ * not depend on it.  No reference counterpart (the reference's CUDA path
+ * fixes its block shapes in GPUupdate.cu:119-202). */
+int gkm_set_sweep_tiles(gkm_ctx *ctx, const int tile[3]);
+int gkm_get_sweep_tiles(const gkm_ctx *ctx, int tile[3]);
+/* Time every tile variant of every sweep on the resident state (reps timed
+ * launches each after one warm-up, the CFL dt of the state) and keep the
+ * fastest per direction.  The resident state is not changed.  tile (may be
+ * NULL) receives the choice, ms (may be NULL, 9 doubles) the average launch
+ * time of variant v of direction d at [3 d + v]. */
+int gkm_autotune(gkm_ctx *ctx, int reps, int tile[3], double ms[9]);
 /* Wait for all work on the context's stream. */
 int gkm_sync(gkm_ctx *ctx);
 /* Flow statistics of the resident state (PrintStats, update.cpp:688-842). */

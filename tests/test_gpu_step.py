@@ -143,3 +143,45 @@ def test_nonphysical_state_reported():
         with pytest.raises(gl.GkmError) as ei:
             c.cfl_dt()
     assert ei.value.code == 4
+
+
+@pytest.mark.parametrize("nc", [32, 48])
+def test_tile_variants_agree_bitwise(nc):
+    """Every sweep tile shape evaluates the same per-face arithmetic, so the
+    step must be bit-for-bit the same whichever shapes are chosen."""
+    W, P, gas = make_case(nc, seed=9)
+    dt = po.cfl_dt(W, 0.5, gas)
+    ref = po.eulerian_step(W, dt, gas, nthreads=8)
+    outs = []
+    for tiles in ((0, 0, 0), (1, 1, 1), (2, 2, 2), (1, 2, 0)):
+        with ctx_for(nc, P) as c:
+            c.set_tiles(tiles)
+            assert c.tiles() == tiles
+            c.upload(W)
+            c.step(dt)
+            outs.append(c.download())
+        e = rel_err(outs[-1], ref)
+        print("tiles", tiles, "rel err", e, flush=True)
+        assert np.all(e < 1e-12), (tiles, e)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_autotune_keeps_state_and_picks_valid_tiles():
+    nc = 48
+    W, P, gas = make_case(nc, seed=10)
+    dt = po.cfl_dt(W, 0.5, gas)
+    ref = po.eulerian_step(W, dt, gas, nthreads=8)
+    with ctx_for(nc, P) as c:
+        c.upload(W)
+        tiles, ms = c.autotune(reps=1)
+        print("autotune", tiles, ms.tolist(), flush=True)
+        assert all(0 <= t < 3 for t in tiles) and c.tiles() == tiles
+        assert np.all(ms > 0)
+        assert np.array_equal(c.download(), W)
+        c.step(dt)
+        out = c.download()
+        with pytest.raises(gl.GkmError) as ei:
+            c.set_tiles((3, 0, 0))
+        assert ei.value.code == 1
+    assert np.all(rel_err(out, ref) < 1e-12)

@@ -1,13 +1,18 @@
-"""Benchmark of the GKM-WENO step on one MI355X.
+"""Benchmark of the GKM-WENO step and the particle tracker on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nc 512] [--precision fp64|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nc 512] [--precision both|mixed|fp64]
+                    [--particles N] [--tiles auto|x,y,z] [--no-cpu-baseline]
 
-A step is one Eulerian GKM-WENO step (CFL dt on the device, x/y/z fused
-sweeps with the update) of the 512^3 periodic cube, state resident in HBM.
-Prints one JSON line: cell-updates/s with the HBM-roofline fraction of the
-dominant kernel (HIP events on the solver's stream) and the CPU baseline
-(the reference's own evolveEulerian when oracle/_ref is built, else the
-oracle restatement) timed on a bounded sample on this host.
+A step is one Eulerian GKM-WENO step (CFL dt on the device, x/y/z fused sweeps
+with the update) of the periodic cube, state resident in HBM.  The workload is
+the 512^3 mixed-precision case (Mt 0.5, Re_lambda 100, k0 4, seed 0) with 2^22
+tracers; the FP64 step of the same state is timed beside it ("FP64 vs mixed").
+Prints one JSON line: cell-updates/s of the mixed step (`value`), the FP64
+figures, the HBM-roofline fraction of the dominant kernel (HIP events on the
+solver's stream), its VALU rate against the reference's operation count,
+particle-updates/s, and the CPU baseline (the reference's own evolveEulerian
+when oracle/_ref is built, else the oracle restatement) timed on a bounded
+sample on this host.
 """
 import argparse
 import json
@@ -22,7 +27,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
 METRIC = "GKM-WENO cell-updates/s at 512^3, FP64 vs mixed, % of HBM roofline"
 # algorithmic HBM bytes per cell of one launch: the x sweep reads the 5 FP64
 # conserved values and writes the 5 FP64 flux-difference sums (40 + 40 B); the
@@ -30,19 +34,27 @@ METRIC = "GKM-WENO cell-updates/s at 512^3, FP64 vs mixed, % of HBM roofline"
 # is 80 B per cell-update (read W^n, write W^{n+1}).
 SWEEP_BYTES = {0: 80.0, 1: 120.0, 2: 120.0}
 STEP_BYTES = 80.0
+# Arithmetic operations of the reference's step with one WENO reconstruction
+# per interface (each +, -, *, / and transcendental counted once), counted
+# through the oracle restatement by oracle/count_ops.cpp: per interface (one
+# sweep launch does one interface per cell) and per cell-update (3 interfaces
+# + the update).
+ALG_OPS_PER_FACE = 8972
+ALG_OPS_PER_CELL_UPDATE = 26951
+KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
+                3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",
+                6: "particle cell sort"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds_cap=30.0):
+def cpu_baseline():
     """Time the CPU Eulerian step on a bounded sample of the same workload
-    (one step of a 128^3 cube of the same initial-condition recipe, ~20 s on
+    (one step of a 128^3 cube of the same initial-condition recipe, ~10-20 s on
     one core)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import ctypes
-
     import pyoracle as po
     from dns_gkm_lpt_amd import ic as icm
 
@@ -87,13 +99,82 @@ def copy_calibration(mib):
     del a, b
 
 
+def kernel_table(ms, cnt):
+    return {KERNEL_NAMES[i]: {"avg_ms": ms[i] / cnt[i], "launches": int(cnt[i])} for i in range(7) if cnt[i]}
+
+
+def run_eulerian(gl, nc, P, W, prec_name, args):
+    """Warm-up, K timed steps, per-kernel event timing; returns (ctx, result)."""
+    prec = gl.GKM_FP64 if prec_name == "fp64" else gl.GKM_MIXED
+    ctx = gl.Context(nc, P, cfl=0.5, precision=prec)
+    ctx.upload(W)
+    if args.tiles == "auto":
+        tiles, tms = ctx.autotune(reps=2)
+        log(f"bench: {prec_name}: autotuned sweep tiles {tiles}; ms per variant (rows x, y, z) "
+            f"{np.round(tms, 2).tolist()}")
+    else:
+        ctx.set_tiles([int(v) for v in args.tiles.split(",")])
+        tiles = ctx.tiles()
+    ctx.run(args.warmup)
+    ctx.sync()
+    t = time.perf_counter()
+    ctx.run(args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t
+    ms_per_step = 1e3 * el / args.steps
+    value = nc ** 3 * args.steps / el
+    log(f"bench: {prec_name}: {args.steps} steps in {el:.3f} s ({ms_per_step:.2f} ms/step)")
+    ctx.timing(True)
+    ctx.run(args.profile_steps)
+    ms, cnt = ctx.timing_read()
+    ctx.timing(False)
+    kern = kernel_table(ms, cnt)
+    log(f"bench: {prec_name}: kernel times {kern}")
+    return ctx, {"value": value, "ms_per_step": ms_per_step, "sweep_tiles": list(tiles), "kernels": kern,
+                 "ms": ms, "cnt": cnt}
+
+
+def run_particles(ctx, npart, steps):
+    """Coupled steps (Eulerian step + spline of the new field + Heun) with npart
+    tracers; the particle share is timed with HIP events per kernel."""
+    from dns_gkm_lpt_amd import ic as icm
+
+    pos = np.ascontiguousarray(icm.uniform_particles(npart, 0)[:, [1, 0, 2]])
+    ctx.lpt_upload(pos)
+    del pos
+    dt = ctx.cfl_dt()
+    ctx.lpt_coupled_step(dt)  # warm-up (builds the first spline)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.lpt_coupled_step(dt)
+    ctx.sync()
+    coupled_ms = 1e3 * (time.perf_counter() - t) / steps
+    ctx.timing(True)
+    for _ in range(steps):
+        ctx.lpt_coupled_step(dt)
+    ms, cnt = ctx.timing_read()
+    ctx.timing(False)
+    kern = kernel_table(ms, cnt)
+    # per step: one spline build, one sample + one Heun launch (slot 5), a sort every 10th step
+    part_ms = (ms[4] + ms[5] + ms[6]) / steps
+    log(f"bench: particles: {npart} tracers, coupled step {coupled_ms:.2f} ms, particle share {part_ms:.2f} ms; "
+        f"{kern}")
+    return {"n": npart, "value": npart / (part_ms * 1e-3), "unit": "particle-updates/s",
+            "ms_per_step": part_ms, "coupled_ms_per_step": coupled_ms,
+            "kernels": {k: v for k, v in kern.items() if not k.startswith("sweep") and k != "cfl reduction"},
+            "note": "particle share = spline prefilter of u, v, w + sample + Heun + amortised cell sort"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nc", type=int, default=512)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="'both': value is the mixed step, the FP64 step of the same state is timed beside it")
+    ap.add_argument("--particles", type=int, default=1 << 22, help="tracers for the particle-update rate (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
     ap.add_argument("--tiles", default="auto",
@@ -105,12 +186,12 @@ def main():
         print("bench.py measures one GPU: the periodic cube is resident on a single MI355X and this build has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.profile_steps < 1 or args.steps < 1:
+        ap.error("--steps and --profile-steps must be >= 1")
 
     from dns_gkm_lpt_amd import _lib as gl
     from dns_gkm_lpt_amd import ic as icm
 
-    if args.profile_steps < 1:
-        ap.error("--profile-steps must be >= 1")
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
     nc = args.nc
@@ -120,68 +201,56 @@ def main():
     W = icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"])
     del ic
     log(f"bench: initial condition {nc}^3 built in {time.perf_counter() - t0:.1f} s")
-    prec = gl.GKM_FP64 if args.precision == "fp64" else gl.GKM_MIXED
-    ctx = gl.Context(nc, P, cfl=0.5, precision=prec)
-    ctx.upload(W)
+
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {}
+    particles = None
+    for prec in precs:
+        ctx, r = run_eulerian(gl, nc, P, W, prec, args)
+        if prec == precs[-1] and args.particles > 0:
+            particles = run_particles(ctx, args.particles, args.steps)
+        ctx.close()
+        res[prec] = r
     del W
-    log("bench: state uploaded")
-    if args.tiles == "auto":
-        tiles, tms = ctx.autotune(reps=2)
-        log(f"bench: autotuned sweep tiles {tiles}; ms per variant (rows x, y, z) {np.round(tms, 2).tolist()}")
-    else:
-        ctx.set_tiles([int(v) for v in args.tiles.split(",")])
-        tiles = ctx.tiles()
-    ctx.run(args.warmup)
-    ctx.sync()
-    log(f"bench: {args.warmup} warm-up steps done")
-
-    ctx.sync()
-    t = time.perf_counter()
-    ctx.run(args.steps)
-    ctx.sync()
-    el = time.perf_counter() - t
-    ms_per_step = 1e3 * el / args.steps
-    value = nc ** 3 * args.steps / el
-    log(f"bench: {args.steps} steps in {el:.3f} s ({ms_per_step:.2f} ms/step)")
-
-    # per-kernel durations, HIP events on the solver's stream
-    ctx.timing(True)
-    ctx.run(args.profile_steps)
-    ms, cnt = ctx.timing_read()
-    ctx.timing(False)
-    names = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
-             3: "cfl reduction"}
-    kern = {names[i]: {"avg_ms": ms[i] / cnt[i], "launches": int(cnt[i])} for i in range(4) if cnt[i]}
+    head = res[precs[-1]]
+    ms, cnt = head["ms"], head["cnt"]
     dom = max(range(3), key=lambda i: ms[i] / max(cnt[i], 1))
     dom_ms = ms[dom] / cnt[dom]
     achieved = SWEEP_BYTES[dom] * nc ** 3 / (dom_ms * 1e-3) / 1e9
-    step_gbs = STEP_BYTES * value / 1e9
-    log(f"bench: kernel times {kern}")
+    alg_tops = ALG_OPS_PER_FACE * nc ** 3 / (dom_ms * 1e-3) / 1e12
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 flux/WENO + f64 state",
+        "dtype": "f64" if precs[-1] == "fp64" else "f32 WENO weights + f32 flux slope terms, f64 state/Euler part",
         "data": "synthetic",
         "config": {"workload": f"GKM-WENO Eulerian step (CFL 0.5 dt on device + fused x/y/z sweeps + update), "
                                f"{nc}^3 periodic cube, Mt 0.5, Re_lambda 100, k0 4, seed 0",
-                   "nc": nc, "precision": args.precision, "sweep_tiles": list(tiles)},
+                   "nc": nc, "precision": precs[-1], "sweep_tiles": head["sweep_tiles"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": names[dom], "avg_ms": dom_ms,
-                     "algorithmic_bytes_per_cell": SWEEP_BYTES[dom]},
-        "step_hbm_fraction": step_gbs / HBM_PEAK_GBS,
-        "kernels": kern,
+                     "kernel": KERNEL_NAMES[dom], "avg_ms": dom_ms,
+                     "algorithmic_bytes_per_cell": SWEEP_BYTES[dom],
+                     "valu": {"alg_ops_per_face": ALG_OPS_PER_FACE, "achieved_tops": alg_tops,
+                              "fp64_peak_tflops": FP64_PEAK_TFLOPS, "frac_of_fp64_peak": alg_tops / FP64_PEAK_TFLOPS,
+                              "note": "reference operation count (one WENO per interface) over the kernel time"}},
+        "step_hbm_fraction": STEP_BYTES * head["value"] / 1e9 / HBM_PEAK_GBS,
+        "kernels": head["kernels"],
     }
-    ctx.close()
+    if "fp64" in res and "mixed" in res:
+        out["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                       "sweep_tiles": res["fp64"]["sweep_tiles"], "kernels": res["fp64"]["kernels"]}
+        out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    if particles is not None:
+        out["particles"] = particles
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -683,7 +683,10 @@ int lpt_heun(gkm_ctx* c, double dt)
     const double* cn[3] = {L.coef[L.cur][0], L.coef[L.cur][1], L.coef[L.cur][2]};
     rc = timed(c, 5, [&] { return lpt::launch_heun(L.np, L.pos, L.U, cn, g, c->dscal + 2, c->stream); });
     if (rc) return rc;
-    if (++L.since_sort >= L.sort_every) HIPCHK(c, lpt::sort_by_cell(L, g, c->dx, c->stream));
+    if (++L.since_sort >= L.sort_every) {
+        rc = timed(c, 6, [&] { return lpt::sort_by_cell(L, g, c->dx, c->stream); });
+        if (rc) return rc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }
@@ -707,7 +710,10 @@ int lpt_coupled_step(gkm_ctx* c, double dt)
     const double* cn[3] = {L.coef[nb][0], L.coef[nb][1], L.coef[nb][2]};
     rc = timed(c, 5, [&] { return lpt::launch_heun(L.np, L.pos, L.U, cn, g, c->dscal, c->stream); });
     if (rc) return rc;
-    if (++L.since_sort >= L.sort_every) HIPCHK(c, lpt::sort_by_cell(L, g, c->dx, c->stream));
+    if (++L.since_sort >= L.sort_every) {
+        rc = timed(c, 6, [&] { return lpt::sort_by_cell(L, g, c->dx, c->stream); });
+        if (rc) return rc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }

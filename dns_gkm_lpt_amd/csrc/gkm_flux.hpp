@@ -229,6 +229,47 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
     }
     const TS gm45 = (TS)(gm4 + gm5), tgm5 = (TS)gm5, tp4 = (TS)p4, tp5 = (TS)p5;
 
+    // ---- equilibrium state, normal-slope terms (E's rest frame, GKMflux.cpp:115-116,
+    // 124-125, 170-171, 210-211).  With FP32 slope terms they go first, so that We and
+    // the cell states are dead before the side blocks; with FP64 slope terms the
+    // compiler allocates registers better when they follow the side blocks.
+    auto normal_terms = [&]() {
+        const TState<TS> st{(TS)E.u, (TS)E.i2l, (TS)(TE(2) * E.lam)};
+        const TS cB = (TS)(gm2 * E.den), cF = (TS)(p2 * E.den);
+        const TE ge = TE(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
+        const TE sl = g_sqrt(E.lam) * E.u;
+        const TE c = TE(2) / (E.den * dx);
+        TS a[5], t[5];
+        {   // u > 0 half with the left normal slope (GKMflux.cpp:115, 124, 170, 210)
+            TS hp[7];
+            const TE h0 = TE(0.5) * g_erfc(-sl);
+            half_tab<7>(st.u, st.th, (TS)h0, (TS)(E.u * h0 + ge), hp);
+            slope_from_lab((We[0] - in.WL(0)) * c, (We[1] - in.WL(1)) * c, (We[2] - in.WL(2)) * c,
+                           (We[3] - in.WL(3)) * c, (We[4] - in.WL(4)) * c, E.v, E.w, st, inv_prn, kp3, ikp3, a);
+#pragma unroll
+            for (int q = 0; q < 5; q++) t[q] = cB * a[q];
+            mom_u(hp + 1, st.th, kp2, kp4, t, Be);
+#pragma unroll
+            for (int q = 0; q < 5; q++) t[q] = cF * a[q];
+            mom_u(hp + 2, st.th, kp2, kp4, t, Fs);
+        }
+        {   // u < 0 half with the right normal slope (GKMflux.cpp:116, 125, 171, 211)
+            TS hm[7];
+            const TE h0 = TE(0.5) * g_erfc(sl);
+            half_tab<7>(st.u, st.th, (TS)h0, (TS)(E.u * h0 - ge), hm);
+            slope_from_lab((in.WR(0) - We[0]) * c, (in.WR(1) - We[1]) * c, (in.WR(2) - We[2]) * c,
+                           (in.WR(3) - We[3]) * c, (in.WR(4) - We[4]) * c, E.v, E.w, st, inv_prn, kp3, ikp3, a);
+#pragma unroll
+            for (int q = 0; q < 5; q++) t[q] = cB * a[q];
+            mom_u(hm + 1, st.th, kp2, kp4, t, Be);
+#pragma unroll
+            for (int q = 0; q < 5; q++) t[q] = cF * a[q];
+            mom_u(hm + 2, st.th, kp2, kp4, t, Fs);
+        }
+    };
+    constexpr bool kNormalFirst = sizeof(TS) == 4;
+    if (kNormalFirst) normal_terms();
+
 #pragma unroll
     for (int side = 0; side < 2; side++) {
         const Maxw<TE>& S = side == 0 ? L : R;
@@ -321,41 +362,14 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         }
     }
 
-    // ---- equilibrium state (its own tangential rest frame)
+    if (!kNormalFirst) normal_terms();
+
+    // ---- equilibrium state, tangential and time-slope terms (its own tangential rest frame)
     {
         const TState<TS> st{(TS)E.u, (TS)E.i2l, (TS)(TE(2) * E.lam)};
         const TS V = (TS)E.v, W = (TS)E.w;
         const TS cB = (TS)(gm2 * E.den), cF = (TS)(p2 * E.den);
-        const TE ge = TE(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
-        const TE sl = g_sqrt(E.lam) * E.u;
-        const TE c = TE(2) / (E.den * dx);
         TS a[5], t[5];
-        {   // u > 0 half with the left normal slope (GKMflux.cpp:115, 124, 170, 210)
-            TS hp[7];
-            const TE h0 = TE(0.5) * g_erfc(-sl);
-            half_tab<7>(st.u, st.th, (TS)h0, (TS)(E.u * h0 + ge), hp);
-            slope_from_lab((We[0] - in.WL(0)) * c, (We[1] - in.WL(1)) * c, (We[2] - in.WL(2)) * c,
-                           (We[3] - in.WL(3)) * c, (We[4] - in.WL(4)) * c, E.v, E.w, st, inv_prn, kp3, ikp3, a);
-#pragma unroll
-            for (int q = 0; q < 5; q++) t[q] = cB * a[q];
-            mom_u(hp + 1, st.th, kp2, kp4, t, Be);
-#pragma unroll
-            for (int q = 0; q < 5; q++) t[q] = cF * a[q];
-            mom_u(hp + 2, st.th, kp2, kp4, t, Fs);
-        }
-        {   // u < 0 half with the right normal slope (GKMflux.cpp:116, 125, 171, 211)
-            TS hm[7];
-            const TE h0 = TE(0.5) * g_erfc(sl);
-            half_tab<7>(st.u, st.th, (TS)h0, (TS)(E.u * h0 - ge), hm);
-            slope_from_lab((in.WR(0) - We[0]) * c, (in.WR(1) - We[1]) * c, (in.WR(2) - We[2]) * c,
-                           (in.WR(3) - We[3]) * c, (in.WR(4) - We[4]) * c, E.v, E.w, st, inv_prn, kp3, ikp3, a);
-#pragma unroll
-            for (int q = 0; q < 5; q++) t[q] = cB * a[q];
-            mom_u(hm + 1, st.th, kp2, kp4, t, Be);
-#pragma unroll
-            for (int q = 0; q < 5; q++) t[q] = cF * a[q];
-            mom_u(hm + 2, st.th, kp2, kp4, t, Fs);
-        }
         TS fu[6];
         full_tab<6>(st.u, st.th, fu);
         // tangential slopes of E (GKMflux.cpp:148-149, 172-173, 212-213); bye, bze are

@@ -119,7 +119,8 @@ int gkm_stats_compute(gkm_ctx *ctx, gkm_stats *out);
  * every sweep/update launch is bracketed by events; gkm_timing_read returns
  * accumulated milliseconds and launch counts for slots
  * 0 = x sweep, 1 = y sweep, 2 = z sweep (fused flux difference + update),
- * 3 = CFL reduction, 4 = spline prefilter, 5 = particle step. */
+ * 3 = CFL reduction, 4 = spline prefilter, 5 = particle sample + Heun step,
+ * 6 = particle cell sort. */
 int gkm_timing_enable(gkm_ctx *ctx, int on);
 int gkm_timing_read(gkm_ctx *ctx, double ms[8], long counts[8]);
 

@@ -237,13 +237,14 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         const TState<TS> st{(TS)E.u, (TS)E.i2l, (TS)(TE(2) * E.lam)};
         const TS cB = (TS)(gm2 * E.den), cF = (TS)(p2 * E.den);
         const TE ge = TE(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
-        const TE sl = g_sqrt(E.lam) * E.u;
+        // erfc(x) + erfc(-x) = 2: the u<0 half's zeroth moment is 1 - that of the u>0 half
+        const TE hp0 = TE(0.5) * g_erfc(-g_sqrt(E.lam) * E.u);
+        const TE hm0 = TE(1) - hp0;
         const TE c = TE(2) / (E.den * dx);
         TS a[5], t[5];
         {   // u > 0 half with the left normal slope (GKMflux.cpp:115, 124, 170, 210)
             TS hp[7];
-            const TE h0 = TE(0.5) * g_erfc(-sl);
-            half_tab<7>(st.u, st.th, (TS)h0, (TS)(E.u * h0 + ge), hp);
+            half_tab<7>(st.u, st.th, (TS)hp0, (TS)(E.u * hp0 + ge), hp);
             slope_from_lab((We[0] - in.WL(0)) * c, (We[1] - in.WL(1)) * c, (We[2] - in.WL(2)) * c,
                            (We[3] - in.WL(3)) * c, (We[4] - in.WL(4)) * c, E.v, E.w, st, inv_prn, kp3, ikp3, a);
 #pragma unroll
@@ -255,8 +256,7 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         }
         {   // u < 0 half with the right normal slope (GKMflux.cpp:116, 125, 171, 211)
             TS hm[7];
-            const TE h0 = TE(0.5) * g_erfc(sl);
-            half_tab<7>(st.u, st.th, (TS)h0, (TS)(E.u * h0 - ge), hm);
+            half_tab<7>(st.u, st.th, (TS)hm0, (TS)(E.u * hm0 - ge), hm);
             slope_from_lab((in.WR(0) - We[0]) * c, (in.WR(1) - We[1]) * c, (in.WR(2) - We[2]) * c,
                            (in.WR(3) - We[3]) * c, (in.WR(4) - We[4]) * c, E.v, E.w, st, inv_prn, kp3, ikp3, a);
 #pragma unroll

@@ -21,8 +21,17 @@ EXPORTS = [
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
     "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
-    "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_coupled_step", "lpt_heun",
+    "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_properties", "lpt_coupled_step", "lpt_heun",
 ]
+
+
+# columns of lpt_properties: the reference's Particle members (ParticleTrack.h:21-22),
+# vectors over (y, x, z), hessP row-major over (y, x, z)
+PROP_NAMES = (["DEN"] + [f"U{a}" for a in range(3)] + ["P"] +
+              [f"{v}{a}" for v in ("gradD", "gradU", "gradV", "gradW", "gradP") for a in range(3)] +
+              ["lapU", "lapV", "lapW"] + [f"hessP{a}" for a in range(9)] +
+              [f"{v}{a}" for v in ("gradLapU", "gradLapV", "gradLapW") for a in range(3)])
+PROP_WIDTH = len(PROP_NAMES)
 
 
 class Config(C.Structure):
@@ -87,6 +96,7 @@ def lib():
         L.lpt_download.argtypes = [P, C.c_void_p, C.c_void_p]
         L.lpt_sample.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.lpt_stats.argtypes = [P, C.POINTER(PStats)]
+        L.lpt_properties.argtypes = [P, _dp]
         L.lpt_coupled_step.argtypes = [P, C.c_double]
         L.lpt_heun.argtypes = [P, C.c_double]
         _lib = L
@@ -239,6 +249,13 @@ class Context:
         s = PStats()
         self._chk(lib().lpt_stats(self._h, C.byref(s)))
         return s
+
+    def lpt_properties(self):
+        """Particle::getParticleProp of every particle: np x 41 (PROP_NAMES columns)."""
+        out = np.empty((self.np, PROP_WIDTH))
+        if self.np:
+            self._chk(lib().lpt_properties(self._h, out))
+        return out
 
     def lpt_coupled_step(self, dt):
         self._chk(lib().lpt_coupled_step(self._h, dt))

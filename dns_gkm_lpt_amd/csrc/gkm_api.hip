@@ -484,7 +484,7 @@ int build_spline(gkm_ctx* c, int b)
     const lpt::Geom g = geom_of(c);
     return timed(c, 4, [&] {
         for (int q = 1; q <= 3; q++) {
-            hipError_t e = lpt::launch_prefilter(c->W[c->cur], c->plane, q, g, c->lpt.pivots, c->lpt.coef[b][q - 1],
+            hipError_t e = lpt::launch_prefilter(c->W[c->cur], c->plane, q, c->gas.gam, g, c->lpt.pivots, c->lpt.coef[b][q - 1],
                                                  c->stream);
             if (e != hipSuccess) return e;
         }
@@ -566,9 +566,9 @@ int lpt_upload(gkm_ctx* c, long np, const double* pos)
     for (long p = 0; p < 3 * np; p++)
         if (!(pos[p] >= 0.0 && pos[p] <= twopi)) return fail(c, GKM_EINVAL, "particle position outside [0, 2 pi]");
     for (void* ptr : {(void*)L.pos, (void*)L.U, (void*)L.G, (void*)L.tag, (void*)L.tag2, (void*)L.pos2, (void*)L.keys,
-                      (void*)L.perm, L.sort_tmp})
+                      (void*)L.perm, L.sort_tmp, (void*)L.props})
         (void)hipFree(ptr);
-    L.pos = L.U = L.G = L.pos2 = nullptr;
+    L.pos = L.U = L.G = L.pos2 = L.props = nullptr;
     L.tag = L.tag2 = L.perm = nullptr;
     L.keys = nullptr;
     L.sort_tmp = nullptr;
@@ -646,6 +646,49 @@ int lpt_sample(gkm_ctx* c, double* U, double* gU, double* gV, double* gW)
                 for (long p = 0; p < L.np; p++)
                     for (int a = 0; a < 3; a++) outs[f][3 * p + a] = G[(size_t)9 * p + 3 * f + a];
     }
+    return GKM_OK;
+}
+
+// splines of rho, p and the velocity Laplacians of the resident state, and the
+// device table of the 8 splines getParticleProp reads (update.cpp:1200-1233)
+static int build_prop_splines(gkm_ctx* c)
+{
+    int rc = ensure_spline(c);
+    if (rc) return rc;
+    lpt::State& L = c->lpt;
+    const long N = c->nc + 8;
+    for (int f = 0; f < 5; f++)
+        if (!L.pcoef[f] && hipMalloc(&L.pcoef[f], sizeof(double) * N * N * N) != hipSuccess)
+            return fail(c, GKM_ENOMEM, "hipMalloc property splines");
+    if (!L.pcs && hipMalloc(&L.pcs, sizeof(double*) * 8) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc spline table");
+    const lpt::Geom g = geom_of(c);
+    HIPCHK(c, lpt::launch_prefilter(c->W[c->cur], c->plane, 0, c->gas.gam, g, L.pivots, L.pcoef[0], c->stream));
+    HIPCHK(c, lpt::launch_prefilter(c->W[c->cur], c->plane, 4, c->gas.gam, g, L.pivots, L.pcoef[1], c->stream));
+    for (int f = 0; f < 3; f++)
+        HIPCHK(c, lpt::launch_lap_spline(L.coef[L.cur][f], g, L.pivots, L.pcoef[2 + f], c->stream));
+    const double* table[8] = {L.pcoef[0], L.coef[L.cur][0], L.coef[L.cur][1], L.coef[L.cur][2],
+                              L.pcoef[1], L.pcoef[2], L.pcoef[3], L.pcoef[4]};
+    HIPCHK(c, hipMemcpyAsync((void*)L.pcs, table, sizeof(table), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // `table` is a host stack array
+    return GKM_OK;
+}
+
+int lpt_properties(gkm_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, GKM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    lpt::State& L = c->lpt;
+    if (L.np == 0) return GKM_OK;
+    int rc = build_prop_splines(c);
+    if (rc) return rc;
+    if (!L.props && hipMalloc(&L.props, sizeof(double) * lpt::kPropWidth * L.np) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc particle properties");
+    const lpt::Geom g = geom_of(c);
+    HIPCHK(c, lpt::launch_props(L.np, L.pos, L.tag, L.pcs, g, L.props, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, L.props, sizeof(double) * lpt::kPropWidth * L.np, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
 }
 

@@ -22,6 +22,9 @@ void release(State& s)
             (void)hipFree(s.coef[b][q]);
             s.coef[b][q] = nullptr;
         }
+    for (int f = 0; f < 5; f++) (void)hipFree(s.pcoef[f]);
+    (void)hipFree((void*)s.pcs);
+    (void)hipFree(s.props);
     (void)hipFree(s.pivots);
     (void)hipFree(s.red);
     (void)hipFree(s.tag);
@@ -65,8 +68,10 @@ __device__ __forceinline__ void line_solve(int M, const Src& src, double* out, l
     out[(long)(M + 1) * st] = 2.0 * cM - xlast;   // xlast = c_{M-1}
 }
 
-// pass 1: z lines straight from the state (velocity = W_q / W_0, periodic wrap)
-__global__ void prefilter_z_kernel(const double* __restrict__ W, long plane, int q, Geom g,
+// pass 1: z lines straight from the state (periodic wrap).  Source field q as
+// generateSplines forms it (update.cpp:1202-1216): 0 = rho, 1..3 = velocity
+// W_q / rho, 4 = pressure (rho (gam-1)) (E/rho - |U|^2/2) with U = W/rho.
+__global__ void prefilter_z_kernel(const double* __restrict__ W, long plane, int q, double gam, Geom g,
                                    const double* __restrict__ piv, double* __restrict__ coef)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,20 +79,30 @@ __global__ void prefilter_z_kernel(const double* __restrict__ W, long plane, int
     const int a = (int)(t / g.nt), b = (int)(t % g.nt);  // padded y, x
     const int nc = g.nc;
     const long base = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc;
-    const double* Wq = W + q * plane;
     auto src = [&](int k) {
         const long c = base + wrapi(k - 3, nc);
-        return Wq[c] / W[c];
+        const double den = W[c];
+        if (q == 0) return den;
+        if (q <= 3) return W[q * plane + c] / den;
+        const double u = W[plane + c] / den, v = W[2 * plane + c] / den, w = W[3 * plane + c] / den;
+        const double Vsqr = u * u + v * v + w * w;
+        return (den * (gam - 1)) * (W[4 * plane + c] / den - 0.5 * Vsqr);
     };
     line_solve(g.nt, src, coef + ((long)a * g.N + b) * g.N, 1, piv);
 }
 
-// pass 2 / 3: in place along x (stride N) or y (stride N^2)
+// in place along z (stride 1; values at the knots of the coefficient box), x
+// (stride N) or y (stride N^2)
 __global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis, const double* __restrict__ piv)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long base, st;
-    if (axis == 1) {  // x lines: a in [0, nt), c in [0, N)
+    if (axis == 0) {  // z lines: a, b in [0, nt)
+        if (t >= (long)g.nt * g.nt) return;
+        const int a = (int)(t / g.nt), b = (int)(t % g.nt);
+        base = ((long)a * g.N + b) * g.N;
+        st = 1;
+    } else if (axis == 1) {  // x lines: a in [0, nt), c in [0, N)
         if (t >= (long)g.nt * g.N) return;
         const int a = (int)(t / g.N), c = (int)(t % g.N);
         base = (long)a * g.N * g.N + c;
@@ -104,21 +119,65 @@ __global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int 
 
 static unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-hipError_t launch_prefilter(const double* W, long plane, int q, const Geom& g, const double* pivots, double* coef,
-                            hipStream_t s)
+static hipError_t prefilter_xy(double* coef, const Geom& g, const double* pivots, hipStream_t s)
 {
-    if (q < 1 || q > 3 || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(prefilter_z_kernel, dim3(blocks_for((long)g.nt * g.nt, 64)), dim3(64), 0, s, W, plane, q, g,
-                       pivots, coef);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(blocks_for((long)g.nt * g.N, 64)), dim3(64), 0, s, coef, g, 1,
                        pivots);
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(blocks_for((long)g.N * g.N, 64)), dim3(64), 0, s, coef, g, 2,
                        pivots);
     return hipGetLastError();
+}
+
+hipError_t launch_prefilter(const double* W, long plane, int q, double gam, const Geom& g, const double* pivots,
+                            double* coef, hipStream_t s)
+{
+    if (q < 0 || q > 4 || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(prefilter_z_kernel, dim3(blocks_for((long)g.nt * g.nt, 64)), dim3(64), 0, s, W, plane, q, gam,
+                       g, pivots, coef);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return prefilter_xy(coef, g, pivots, s);
+}
+
+// Laplacian of the spline c at every padded knot (eval_UBspline_3d_d_vgl at the
+// cell centres, update.cpp:1218-1229).  At a knot the uniform cubic B-spline
+// weights are (1, 4, 1)/6 for the value and (1, -2, 1) delta^-2 for the second
+// derivative, so the Laplacian is a 27-point stencil on the coefficients.
+// The result lands at the knot positions of the coefficient box `lap`.
+__global__ void lap_knots_kernel(const double* __restrict__ c, Geom g, double* __restrict__ lap)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nt = g.nt, N = g.N;
+    if (t >= nt * nt * nt) return;
+    const int a = (int)(t / (nt * nt)), b = (int)((t / nt) % nt), k = (int)(t % nt);
+    const double w[3] = {1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0}, d2[3] = {1.0, -2.0, 1.0};
+    double hyy = 0, hxx = 0, hzz = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const double* row = c + ((long)(a + i) * N + (b + j)) * N + k;
+            const double rv = w[0] * row[0] + w[1] * row[1] + w[2] * row[2];
+            const double r2 = d2[0] * row[0] + d2[1] * row[1] + d2[2] * row[2];
+            hyy += d2[i] * w[j] * rv;
+            hxx += w[i] * d2[j] * rv;
+            hzz += w[i] * w[j] * r2;
+        }
+    lap[((long)a * N + b) * N + k] = (hyy + hxx + hzz) * (g.dinv * g.dinv);
+}
+
+hipError_t launch_lap_spline(const double* c, const Geom& g, const double* pivots, double* lap, hipStream_t s)
+{
+    if (c == lap || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
+    const long n = (long)g.nt * g.nt * g.nt;
+    hipLaunchKernelGGL(lap_knots_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, c, g, lap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(blocks_for((long)g.nt * g.nt, 64)), dim3(64), 0, s, lap, g, 0,
+                       pivots);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return prefilter_xy(lap, g, pivots, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -129,6 +188,7 @@ hipError_t launch_prefilter(const double* W, long plane, int q, const Geom& g, c
 // ---------------------------------------------------------------------------
 struct Loc {
     int i;
+    double t;
     double w[4], d[4];
 };
 
@@ -142,6 +202,7 @@ __device__ __forceinline__ Loc locate(double p, const Geom& g, bool deriv)
     if (i > g.N - 4) { i = g.N - 4; t = 1.0; }
     Loc L;
     L.i = i;
+    L.t = t;
     const double t2 = t * t, t3 = t2 * t;
     const double s6 = 1.0 / 6.0;
     L.w[0] = (-t3 + 3 * t2 - 3 * t + 1) * s6;
@@ -200,6 +261,95 @@ hipError_t launch_sample(long np, const double* pos, const double* const cu[3], 
 {
     if (np <= 0) return hipSuccess;
     hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, cu[0], cu[1], cu[2], g, U, G);
+    return hipGetLastError();
+}
+
+// Particle::getParticleProp (ParticleTrack.cpp:99-113): value + gradient of rho,
+// u, v, w and of the three velocity Laplacians, value + gradient + Hessian of p,
+// at each particle.  cs = {rho, u, v, w, p, lap u, lap v, lap w} splines.  Row
+// tag[p] of out receives the kPropWidth members in the order of the reference's
+// Particle (ParticleTrack.h:21-22): DEN, U[3], P, gradD[3], gradU[3], gradV[3],
+// gradW[3], gradP[3], lapU, lapV, lapW, hessP[9], gradLapU[3], gradLapV[3],
+// gradLapW[3]; vectors over (y, x, z), hessP row-major over (y, x, z).
+__global__ __launch_bounds__(256) void props_kernel(long np, const double* __restrict__ pos, const int* __restrict__ tag,
+                                                    const double* const* __restrict__ cs, Geom g,
+                                                    double* __restrict__ out)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const Loc Ly = locate(pos[3 * p + 0], g, true);
+    const Loc Lx = locate(pos[3 * p + 1], g, true);
+    const Loc Lz = locate(pos[3 * p + 2], g, true);
+    // second-derivative weights of the cubic B-spline
+    double hy[4], hx[4], hz[4];
+    {
+        const double d2 = g.dinv * g.dinv;
+        const Loc* L3[3] = {&Ly, &Lx, &Lz};
+        double* H3[3] = {hy, hx, hz};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double t = L3[a]->t;
+            H3[a][0] = (1 - t) * d2;
+            H3[a][1] = (3 * t - 2) * d2;
+            H3[a][2] = (1 - 3 * t) * d2;
+            H3[a][3] = t * d2;
+        }
+    }
+    const long N = g.N;
+    double* o = out + (long)tag[p] * kPropWidth;
+    // output slots of value and gradient per field
+    const int vslot[8] = {0, 1, 2, 3, 4, 20, 21, 22};
+    const int gslot[8] = {5, 8, 11, 14, 17, 32, 35, 38};
+    for (int f = 0; f < 8; f++) {
+        double v = 0, gy = 0, gx = 0, gz = 0, hyy = 0, hyx = 0, hyz = 0, hxx = 0, hxz = 0, hzz = 0;
+        const double* cf = cs[f];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                const double* row = cf + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
+                double rv = 0, rd = 0, rh = 0;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    rv += Lz.w[c] * row[c];
+                    rd += Lz.d[c] * row[c];
+                    rh += hz[c] * row[c];
+                }
+                const double wab = Ly.w[a] * Lx.w[b];
+                v += wab * rv;
+                gz += wab * rd;
+                gy += Ly.d[a] * Lx.w[b] * rv;
+                gx += Ly.w[a] * Lx.d[b] * rv;
+                if (f == 4) {
+                    hyy += hy[a] * Lx.w[b] * rv;
+                    hxx += Ly.w[a] * hx[b] * rv;
+                    hzz += wab * rh;
+                    hyx += Ly.d[a] * Lx.d[b] * rv;
+                    hyz += Ly.d[a] * Lx.w[b] * rd;
+                    hxz += Ly.w[a] * Lx.d[b] * rd;
+                }
+            }
+        o[vslot[f]] = v;
+        o[gslot[f] + 0] = gy;
+        o[gslot[f] + 1] = gx;
+        o[gslot[f] + 2] = gz;
+        if (f == 4) {
+            o[23] = hyy;
+            o[24] = hyx;
+            o[25] = hyz;
+            o[26] = hyx;
+            o[27] = hxx;
+            o[28] = hxz;
+            o[29] = hyz;
+            o[30] = hxz;
+            o[31] = hzz;
+        }
+    }
+}
+
+hipError_t launch_props(long np, const double* pos, const int* tag, const double* const* cs_dev, const Geom& g,
+                        double* out, hipStream_t s)
+{
+    if (np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(props_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, tag, cs_dev, g, out);
     return hipGetLastError();
 }
 

@@ -12,6 +12,10 @@ struct State {
     double* U = nullptr;       // np x 3 velocity sampled at the start of the step
     double* G = nullptr;       // np x 9 velocity gradients gU, gV, gW over (y, x, z)
     double* coef[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // (nc+8)^3 each
+    // full-property splines of the resident state {rho, p, lap u, lap v, lap w}, built on demand
+    double* pcoef[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const double** pcs = nullptr;  // device array of the 8 spline pointers for props_kernel
+    double* props = nullptr;       // np x kPropWidth, tag order
     int cur = 0;               // coef[cur] = spline of the resident state when valid
     bool valid = false;
     double* pivots = nullptr;  // Thomas pivots 1/(4 - p_{r-1}) of the [1 4 1] system
@@ -39,10 +43,18 @@ struct Geom {
     double start, dinv; // 1/delta
 };
 
-// velocity component q (1..3) of the unpadded state W -> coefficients of its
-// natural tricubic spline on the padded grid (periodic ghosts read by wrap).
-hipError_t launch_prefilter(const double* W, long plane, int q, const Geom& g, const double* pivots, double* coef,
-                            hipStream_t s);
+// field q of the unpadded state W (0 = rho, 1..3 = velocity component,
+// 4 = pressure) -> coefficients of its natural tricubic spline on the padded
+// grid (periodic ghosts read by wrap).
+hipError_t launch_prefilter(const double* W, long plane, int q, double gam, const Geom& g, const double* pivots,
+                            double* coef, hipStream_t s);
+// spline of the Laplacian of spline c (knot values, then prefilter in place)
+hipError_t launch_lap_spline(const double* c, const Geom& g, const double* pivots, double* lap, hipStream_t s);
+// full particle properties, kPropWidth doubles per particle written to row tag[p];
+// cs_dev is a device array of the 8 spline pointers {rho, u, v, w, p, lap u, lap v, lap w}
+constexpr int kPropWidth = 41;
+hipError_t launch_props(long np, const double* pos, const int* tag, const double* const* cs_dev, const Geom& g,
+                        double* out, hipStream_t s);
 hipError_t launch_sample(long np, const double* pos, const double* const cu[3], const Geom& g, double* U, double* G,
                          hipStream_t s);
 hipError_t launch_heun(long np, double* pos, const double* U, const double* const cn[3], const Geom& g,

@@ -147,6 +147,16 @@ int lpt_download(gkm_ctx *ctx, double *pos, double *vel);
 int lpt_sample(gkm_ctx *ctx, double *U, double *gU, double *gV, double *gW);
 /* Particle statistics from the last sample (GetParticleStats). */
 int lpt_stats(gkm_ctx *ctx, gkm_pstats *out);
+/* Particle::getParticleProp (ParticleTrack.cpp:99-113) of every particle on
+ * the resident state: splines of rho, u, v, w, p and of the Laplacians of u,
+ * v, w (generateSplines, update.cpp:1200-1233), evaluated at the particle.
+ * out: np x 41 doubles in upload order, the members of the reference's
+ * Particle in declaration order (ParticleTrack.h:21-22): DEN, U[3], P,
+ * gradD[3], gradU[3], gradV[3], gradW[3], gradP[3], lapU, lapV, lapW,
+ * hessP[9], gradLapU[3], gradLapV[3], gradLapW[3]; vectors over (y, x, z),
+ * hessP row-major over (y, x, z). */
+#define LPT_PROP_WIDTH 41
+int lpt_properties(gkm_ctx *ctx, double *out);
 /* One coupled step of the Evolve loop (update.cpp:124-170) with the resident
  * state: sample U at x from the current field, advance the field by dt
  * (gkm_step), rebuild the velocity spline, Heun-advance the particles

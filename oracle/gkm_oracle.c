@@ -650,6 +650,78 @@ void orc_velocity_coefs(int nc, const double *W, double *cu, double *cv, double 
     free(f);
 }
 
+/* Laplacian of spline `coefs` at every padded knot, out: nt^3 values over
+ * (y, x, z) -- eval_UBspline_3d_d_vgl at the cell centres y = (j-3)dx + dx/2
+ * etc. (update.cpp:1218-1229).  At a knot the fractional part is 0, where the
+ * B-spline weights are w = (1, 4, 1, 0)/6 and w'' = (1, -2, 1, 0)/dx^2; the
+ * fourth coefficient has weight 0 (at the last knot the reference's unclamped
+ * index reads one element past the coefficient line there, times 0).        */
+void orc_lap_field(int nc, const double *coefs, double dx, double *lap)
+{
+    const int nt = nc + 6, N = nt + 2;
+    const double w[3] = {1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0}, d2[3] = {1.0, -2.0, 1.0};
+    const double dinv2 = 1.0 / (dx * dx);
+    for (int a = 0; a < nt; a++)
+        for (int b = 0; b < nt; b++)
+            for (int k = 0; k < nt; k++) {
+                double hyy = 0, hxx = 0, hzz = 0;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++)
+                        for (int l = 0; l < 3; l++) {
+                            const double P = coefs[((long)(a + i) * N + (b + j)) * N + (k + l)];
+                            hyy += d2[i] * w[j] * w[l] * P;
+                            hxx += w[i] * d2[j] * w[l] * P;
+                            hzz += w[i] * w[j] * d2[l] * P;
+                        }
+                lap[((long)a * nt + b) * nt + k] = (hyy + hxx + hzz) * dinv2;
+            }
+}
+
+/* generateSplines(W, splinep, ...), update.cpp:1200-1233: splines of rho, u,
+ * v, w, p and of the Laplacians of u, v, w; then Particle::getParticleProp
+ * (ParticleTrack.cpp:99-113) at each particle.  out: np x 41, the Particle
+ * members DEN, U[3], P, gradD[3], gradU[3], gradV[3], gradW[3], gradP[3],
+ * lapU, lapV, lapW, hessP[9], gradLapU[3], gradLapV[3], gradLapW[3].         */
+void orc_particle_props(int nc, const double *W, double gam, double dx, long np, const double *pos,
+                        double *out)
+{
+    const int nt = nc + 6, N = nt + 2;
+    const long Nt = (long)nt * nt * nt, NN = (long)N * N * N;
+    double *f = (double *)malloc(sizeof(double) * Nt);
+    double *cs[8];
+    for (int s = 0; s < 8; s++) cs[s] = (double *)malloc(sizeof(double) * NN);
+    /* 0 rho, 1..3 u, v, w, 4 p (update.cpp:1202-1210) */
+    for (int q = 0; q < 5; q++) {
+        for (long c = 0; c < Nt; c++) {
+            const double den = W[c];
+            if (q == 0)
+                f[c] = den;
+            else if (q <= 3)
+                f[c] = W[q * Nt + c] / den;
+            else {
+                const double u = W[Nt + c] / den, v = W[2 * Nt + c] / den, w = W[3 * Nt + c] / den;
+                const double Vsqr = u * u + v * v + w * w;
+                f[c] = (den * (gam - 1)) * (W[4 * Nt + c] / den - 0.5 * Vsqr);
+            }
+        }
+        orc_spline_coefs(nc, f, cs[q]);
+    }
+    for (int q = 0; q < 3; q++) {
+        orc_lap_field(nc, cs[1 + q], dx, f);
+        orc_spline_coefs(nc, f, cs[5 + q]);
+    }
+    const int vslot[8] = {0, 1, 2, 3, 4, 20, 21, 22};
+    const int gslot[8] = {5, 8, 11, 14, 17, 32, 35, 38};
+    for (long p = 0; p < np; p++) {
+        const double *x = pos + 3 * p;
+        double *o = out + 41 * p;
+        for (int s = 0; s < 8; s++)
+            orc_spline_eval(nc, cs[s], dx, x[0], x[1], x[2], o + vslot[s], o + gslot[s], s == 4 ? o + 23 : NULL);
+    }
+    for (int s = 0; s < 8; s++) free(cs[s]);
+    free(f);
+}
+
 /* Particle::getParticleProp velocity subset, ParticleTrack.cpp:99-113:
  * pos = (y, x, z); U = (u, v, w); gU/gV/gW = gradients over (y, x, z).       */
 void orc_particle_sample(int nc, const double *cu, const double *cv, const double *cw, double dx, long np,

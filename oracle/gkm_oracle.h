@@ -66,6 +66,9 @@ void orc_particle_sample(int nc, const double *cu, const double *cv, const doubl
                          double *gV, double *gW);
 void orc_particle_heun(int nc, const double *cu, const double *cv, const double *cw,
                        double dx, long np, double *pos, const double *U, double dt);
+void orc_lap_field(int nc, const double *coefs, double dx, double *lap);
+void orc_particle_props(int nc, const double *W, double gam, double dx, long np, const double *pos,
+                        double *out);
 
 /* --- diagnostics --------------------------------------------------------- */
 typedef struct {

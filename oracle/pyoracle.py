@@ -76,6 +76,8 @@ def lib():
                                           _dp, _dp, _dp]
         L.orc_particle_heun.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_long, _dp, _dp,
                                         C.c_double]
+        L.orc_lap_field.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.orc_particle_props.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_long, _dp, _dp]
         L.orc_field_stats.argtypes = [C.c_int, _dp, _dp, _dp, _dp, G, C.c_double, C.c_double,
                                       C.c_double, C.POINTER(Stats)]
         L.orc_particle_stats.argtypes = [C.c_long, C.c_long, _dp, _dp, _dp, _dp, C.POINTER(PStats)]
@@ -188,6 +190,23 @@ def particle_sample(cs, dx, pos):
     U, gU, gV, gW = (np.empty((n, 3)) for _ in range(4))
     lib().orc_particle_sample(nc, cs[0], cs[1], cs[2], dx, n, pos, U, gU, gV, gW)
     return U, gU, gV, gW
+
+
+def lap_field(coefs, dx):
+    nc = coefs.shape[-1] - 8
+    out = np.empty((nc + 6,) * 3)
+    lib().orc_lap_field(nc, np.ascontiguousarray(coefs), dx, out)
+    return out
+
+
+def particle_props(W, gam, dx, pos):
+    """np x 41 Particle members (getParticleProp) from padded W."""
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    nc = W.shape[-1] - 6
+    out = np.empty((pos.shape[0], 41))
+    lib().orc_particle_props(nc, W, gam, dx, pos.shape[0], pos, out)
+    return out
 
 
 def particle_heun(cs, dx, pos, U, dt):

@@ -89,3 +89,21 @@ def test_particle_stats_match_oracle():
         s = c.lpt_stats()
     for a, b in ((s.tke, ref.tke), (s.sijsij, ref.sijsij), (s.skew_u, ref.skew_u), (s.skew_w, ref.skew_w)):
         assert abs(a - b) <= 1e-10 * abs(b), (a, b)
+
+
+def test_particle_properties_match_oracle():
+    """lpt_properties (getParticleProp with all eight splines) against the oracle,
+    column by column: within 1e-10 of the column's largest magnitude."""
+    nc = 32
+    W, P, gas = make_case(nc, seed=5)
+    pos = particles(2000, 5)
+    ref = po.particle_props(W, P["gam"], gas.dx, pos)
+    with gl.Context(nc, P) as c:
+        c.upload(W)
+        c.lpt_upload(pos)
+        out = c.lpt_properties()
+    assert out.shape == (2000, gl.PROP_WIDTH) == ref.shape
+    err = np.max(np.abs(out - ref), axis=0) / np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
+    worst = int(np.argmax(err))
+    print("props worst column", gl.PROP_NAMES[worst], err[worst], flush=True)
+    assert np.all(err < 1e-10), dict(zip(gl.PROP_NAMES, err))

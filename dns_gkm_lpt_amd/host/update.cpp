@@ -1,18 +1,22 @@
 // update.cpp -- the reference's time loop (update.cpp:39-188) driving the
 // resident GPU solver through the C ABI.
 //
-// Per iteration, as the reference orders it: particle properties from the
-// field at t^n (getParticleProp), particle statistics and, every 10th
-// iteration, the statistics line (PrintStats); the Eulerian step; the periodic
-// halo; the velocity spline of t^{n+1}; the Heun particle step; the new CFL dt.
-// On the GPU all of it after the first upload stays on the device; the host
-// sees the statistics and, at the end, the state and the particles.
+// Per iteration, as the reference orders it: every SAVE_INTERVAL iterations the
+// field and restart files (writeField, WriteReInit); every 10th iteration the
+// particle properties from the field at t^n (getParticleProp), the particle
+// file (writeParticleData), the particle statistics and the statistics line
+// (PrintStats); the Eulerian step; the periodic halo; the velocity spline of
+// t^{n+1}; the Heun particle step; the new CFL dt.  On the GPU all of it after
+// the first upload stays on the device; the host sees the statistics, the
+// output files and, at the end, the state and the particles.  DNSGKM_OUTPUT=0
+// turns the output files off.
 #include "update.h"
 
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "dnsgkm_host.h"
@@ -155,9 +159,24 @@ void Evolve(ptype *W[5], MPI_Comm comm3d)
     iter = 0;
     long nmax = -1;  // a fixed step count instead of t <= tmax
     if (const char *e = std::getenv("DNSGKM_NSTEPS")) nmax = std::atol(e);
+    bool output = true;
+    if (const char *e = std::getenv("DNSGKM_OUTPUT")) output = std::string(e) != "0";
     while (t <= tmax && (nmax < 0 || iter < nmax)) {
+        if (output && iter % SAVE_INTERVAL == 0) {
+            dnsgkm_check(gkm_download_state(c, W), "writeField download");
+            std::vector<ptype> prim(5 * (size_t)Nt_seg);
+            ptype *P[5];
+            FOR(q, 5) P[q] = prim.data() + (size_t)q * Nt_seg;
+            generatePrimitives(W, P[0], P[1], P[2], P[3], P[4]);
+            writeField(P[0], P[1], P[2], P[3], P[4]);
+            WriteReInit(W);
+        }
         if (iter % 10 == 0) {
             if (count > 0) {
+                if (output) {
+                    getParticleProps(particles);
+                    writeParticleData(particles);
+                }
                 dnsgkm_check(lpt_sample(c, nullptr, nullptr, nullptr, nullptr), "getParticleProp");
                 GetParticleStats(particles);
             }

@@ -31,5 +31,16 @@ void locateParticles(Particle *particles);
 void sampleParticles(Particle *particles);
 // particle statistics of the last sample (GetParticleStats, update.cpp:1111-1162)
 void GetParticleStats(Particle *particles);
+// every member of every particle on the current field (getParticleProp,
+// ParticleTrack.cpp:99-113, with the eight splines of generateSplines)
+void getParticleProps(Particle *particles);
+// one particle's members from pos (y, x, z) and an lpt_properties row
+void fillParticleProps(Particle &q, const double *pos, const double *props);
+
+// output files (output.cpp; update.cpp:1324-1422)
+void generatePrimitives(ptype *W[5], ptype *den, ptype *u, ptype *v, ptype *w, ptype *p);
+void writeField(ptype *den, ptype *u, ptype *v, ptype *w, ptype *p);
+void writeParticleData(Particle *particles);
+void WriteReInit(ptype *W[5]);
 
 #endif

@@ -146,9 +146,11 @@ def test_nonphysical_state_reported():
 
 
 @pytest.mark.parametrize("nc", [32, 48])
-def test_tile_variants_agree_bitwise(nc):
-    """Every sweep tile shape evaluates the same per-face arithmetic, so the
-    step must be bit-for-bit the same whichever shapes are chosen."""
+def test_tile_variants_agree(nc):
+    """Every sweep tile shape evaluates the same per-face formulas; each
+    instantiation is compiled on its own (the compiler may fuse multiply-adds
+    differently), so the shapes agree to rounding: 1e-14 normwise per
+    variable, and each matches the oracle to 1e-12."""
     W, P, gas = make_case(nc, seed=9)
     dt = po.cfl_dt(W, 0.5, gas)
     ref = po.eulerian_step(W, dt, gas, nthreads=8)
@@ -164,7 +166,9 @@ def test_tile_variants_agree_bitwise(nc):
         print("tiles", tiles, "rel err", e, flush=True)
         assert np.all(e < 1e-12), (tiles, e)
     for o in outs[1:]:
-        assert np.array_equal(o, outs[0])
+        d = rel_err(o, outs[0])
+        print("max difference between tile variants", d, flush=True)
+        assert np.all(d < 1e-14), d
 
 
 def test_autotune_keeps_state_and_picks_valid_tiles():

@@ -41,6 +41,20 @@ STEP_BYTES = 80.0
 # + the update).
 ALG_OPS_PER_FACE = 8972
 ALG_OPS_PER_CELL_UPDATE = 26951
+# HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (each scaled by what the counter reads for a plain 1 GiB copy:
+# FETCH x 2.0, WRITE x 1.0), GiB read + GiB written, keyed by
+# (precision, direction, tile variant, nc); the commands and the table are in
+# profiles/sweep_traffic_512.md.  Used for roofline.traffic when the dominant
+# kernel of the run is one of these configurations.
+MEASURED_TRAFFIC_GIB = {
+    ("mixed", 0, 2, 512): (10.87, 5.00),
+    ("mixed", 1, 2, 512): (12.56, 5.00),
+    ("mixed", 2, 0, 512): (20.86, 5.00),
+    ("fp64", 0, 0, 512): (59.05, 29.28),
+    ("fp64", 1, 2, 512): (70.76, 27.62),
+    ("fp64", 2, 0, 512): (75.65, 29.16),
+}
 KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
                 3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",
                 6: "particle cell sort"}
@@ -218,6 +232,8 @@ def main():
     dom_ms = ms[dom] / cnt[dom]
     achieved = SWEEP_BYTES[dom] * nc ** 3 / (dom_ms * 1e-3) / 1e9
     alg_tops = ALG_OPS_PER_FACE * nc ** 3 / (dom_ms * 1e-3) / 1e12
+    measured = MEASURED_TRAFFIC_GIB.get((precs[-1], dom, head["sweep_tiles"][dom], nc))
+    traffic = None if measured is None else (measured[0] + measured[1]) * 2.0 ** 30
 
     out = {
         "metric": METRIC,
@@ -236,7 +252,10 @@ def main():
                                f"{nc}^3 periodic cube, Mt 0.5, Re_lambda 100, k0 4, seed 0",
                    "nc": nc, "precision": precs[-1], "sweep_tiles": head["sweep_tiles"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
+                                     "calibrated on a 1 GiB copy; profiles/sweep_traffic_512.md)"
+                                     if traffic else "no counter measurement for this configuration",
                      "kernel": KERNEL_NAMES[dom], "avg_ms": dom_ms,
                      "algorithmic_bytes_per_cell": SWEEP_BYTES[dom],
                      "valu": {"alg_ops_per_face": ALG_OPS_PER_FACE, "achieved_tops": alg_tops,

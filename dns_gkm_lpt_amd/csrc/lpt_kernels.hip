@@ -68,27 +68,94 @@ __device__ __forceinline__ void line_solve(int M, const Src& src, double* out, l
     out[(long)(M + 1) * st] = 2.0 * cM - xlast;   // xlast = c_{M-1}
 }
 
-// pass 1: z lines straight from the state (periodic wrap).  Source field q as
+// Pass 1: z lines straight from the state (periodic wrap).  Source field q as
 // generateSplines forms it (update.cpp:1202-1216): 0 = rho, 1..3 = velocity
 // W_q / rho, 4 = pressure (rho (gam-1)) (E/rho - |U|^2/2) with U = W/rho.
-__global__ void prefilter_z_kernel(const double* __restrict__ W, long plane, int q, double gam, Geom g,
-                                   const double* __restrict__ piv, double* __restrict__ coef)
+// Field q of the state at padded position (a, b, k) of the knot grid.
+__device__ __forceinline__ double field_at(const double* __restrict__ W, long plane, int q, double gam, int nc, int a,
+                                          int b, int k)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)g.nt * g.nt) return;
-    const int a = (int)(t / g.nt), b = (int)(t % g.nt);  // padded y, x
-    const int nc = g.nc;
-    const long base = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc;
-    auto src = [&](int k) {
-        const long c = base + wrapi(k - 3, nc);
-        const double den = W[c];
-        if (q == 0) return den;
-        if (q <= 3) return W[q * plane + c] / den;
-        const double u = W[plane + c] / den, v = W[2 * plane + c] / den, w = W[3 * plane + c] / den;
-        const double Vsqr = u * u + v * v + w * w;
-        return (den * (gam - 1)) * (W[4 * plane + c] / den - 0.5 * Vsqr);
-    };
-    line_solve(g.nt, src, coef + ((long)a * g.N + b) * g.N, 1, piv);
+    const long c = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc + wrapi(k - 3, nc);
+    const double den = W[c];
+    if (q == 0) return den;
+    if (q <= 3) return W[q * plane + c] / den;
+    const double u = W[plane + c] / den, v = W[2 * plane + c] / den, w = W[3 * plane + c] / den;
+    const double Vsqr = u * u + v * v + w * w;
+    return (den * (gam - 1)) * (W[4 * plane + c] / den - 0.5 * Vsqr);
+}
+
+// Pass 1 staged through LDS: one wave solves the z lines of 64 consecutive x
+// knots (one line per lane) in chunks of KC positions.  Each chunk is loaded
+// and stored row by row (64 consecutive z values per row: coalesced), and each
+// lane runs the recurrence of line_solve on its own row in LDS: the same
+// operations in the same order as line_solve.
+template <int KC>
+__global__ __launch_bounds__(64) void prefilter_z_lds_kernel(const double* __restrict__ W, long plane, int q,
+                                                             double gam, Geom g, const double* __restrict__ piv,
+                                                             double* __restrict__ coef)
+{
+    __shared__ double tile[64][KC + 1];
+    const int t = threadIdx.x;
+    const int ngb = (g.nt + 63) / 64;
+    const int a = blockIdx.x / ngb, b0 = (blockIdx.x % ngb) * 64;
+    const int nl = g.nt - b0 < 64 ? g.nt - b0 : 64;  // lines in this group
+    const int nc = g.nc, M = g.nt, n = M - 2;
+    const long N = g.N;
+    const bool mine = t < nl;
+    double* row0 = coef + ((long)a * N + b0) * N;   // line l of the group at row0 + l N
+    double c1 = 0.0, cM = 0.0;
+    if (mine) {
+        c1 = field_at(W, plane, q, gam, nc, a, b0 + t, 0);
+        cM = field_at(W, plane, q, gam, nc, a, b0 + t, M - 1);
+    }
+    // forward sweep: f_{r+1} -> dp_r at position r + 2
+    double dp = 0.0;
+    for (int r0 = 0; r0 < n; r0 += KC) {
+        const int kc = n - r0 < KC ? n - r0 : KC;
+        __syncthreads();
+        if (t < kc)
+            for (int l = 0; l < nl; l++) tile[l][t] = field_at(W, plane, q, gam, nc, a, b0 + l, r0 + 1 + t);
+        __syncthreads();
+        if (mine)
+            for (int j = 0; j < kc; j++) {
+                const int r = r0 + j;
+                double rhs = 6.0 * tile[t][j];
+                if (r == 0) rhs -= c1;
+                if (r == n - 1) rhs -= cM;
+                dp = (rhs - dp) * piv[r];
+                tile[t][j] = dp;
+            }
+        __syncthreads();
+        if (t < kc)
+            for (int l = 0; l < nl; l++) row0[(long)l * N + r0 + 2 + t] = tile[l][t];
+    }
+    // backward sweep, chunks from the end: x_r = dp_r - piv[r] x_{r+1}
+    double x = dp;
+    const double xlast = x;
+    for (int r0 = ((n - 1) / KC) * KC; r0 >= 0; r0 -= KC) {
+        const int kc = n - r0 < KC ? n - r0 : KC;
+        __syncthreads();
+        if (t < kc)
+            for (int l = 0; l < nl; l++) tile[l][t] = row0[(long)l * N + r0 + 2 + t];
+        __syncthreads();
+        if (mine)
+            for (int j = kc - 1; j >= 0; j--) {
+                const int r = r0 + j;
+                if (r == n - 1) continue;  // c_{n+1} = dp_{n-1}
+                x = tile[t][j] - piv[r] * x;
+                tile[t][j] = x;
+            }
+        __syncthreads();
+        if (t < kc)
+            for (int l = 0; l < nl; l++) row0[(long)l * N + r0 + 2 + t] = tile[l][t];
+    }
+    if (mine) {
+        double* out = row0 + (long)t * N;
+        out[1] = c1;
+        out[M] = cM;
+        out[0] = 2.0 * c1 - x;         // x = c_2
+        out[M + 1] = 2.0 * cM - xlast;  // xlast = c_{M-1}
+    }
 }
 
 // in place along z (stride 1; values at the knots of the coefficient box), x
@@ -134,8 +201,8 @@ hipError_t launch_prefilter(const double* W, long plane, int q, double gam, cons
                             double* coef, hipStream_t s)
 {
     if (q < 0 || q > 4 || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(prefilter_z_kernel, dim3(blocks_for((long)g.nt * g.nt, 64)), dim3(64), 0, s, W, plane, q, gam,
-                       g, pivots, coef);
+    const unsigned groups = (unsigned)(g.nt * ((g.nt + 63) / 64));
+    hipLaunchKernelGGL(prefilter_z_lds_kernel<64>, dim3(groups), dim3(64), 0, s, W, plane, q, gam, g, pivots, coef);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return prefilter_xy(coef, g, pivots, s);

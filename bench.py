@@ -180,6 +180,42 @@ def run_particles(ctx, npart, steps):
             "note": "particle share = spline prefilter of u, v, w + sample + Heun + amortised cell sort"}
 
 
+def run_ensemble(gl, steps, warmup):
+    """The batched-ensemble configuration: 64 independent realizations (24 at
+    32^3, 24 at 48^3, 16 at 64^3), each with its own seed, k0 in [3, 6], Mt in
+    [0.1, 0.6], Re_lambda in [20, 60], mixed precision, tiles autotuned; steps
+    replayed from captured HIP graphs, timed beside the same steps launched
+    eagerly."""
+    from dns_gkm_lpt_amd import ic as icm
+
+    rng = np.random.default_rng(4)
+    ncs = [32] * 24 + [48] * 24 + [64] * 16
+    cases = []
+    for m, nc in enumerate(ncs):
+        k0, Mt, Re = rng.uniform(3, 6), rng.uniform(0.1, 0.6), rng.uniform(20, 60)
+        ic = icm.make_ic(nc, k0=k0, Mt=Mt, Re=Re, seed=100 + m)
+        cases.append((nc, ic["params"], icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"])))
+    cells = sum(nc ** 3 for nc in ncs)
+    out = {"members": len(ncs), "grids": "24 x 32^3 + 24 x 48^3 + 16 x 64^3", "precision": "mixed"}
+    with gl.Ensemble(ncs, [c[1] for c in cases], cfl=0.5, precision=gl.GKM_MIXED) as e:
+        for m, c in enumerate(cases):
+            e.members[m].upload(c[2])
+        tiles, _ = e.autotune(reps=2)
+        e.run(warmup, graph=True)
+        for mode, graph in (("graph", True), ("eager", False)):
+            t = time.perf_counter()
+            e.run(steps, graph=graph)
+            el = time.perf_counter() - t
+            out[f"ms_per_step_{mode}"] = 1e3 * el / steps
+            out[f"member_cell_updates_per_s_{mode}"] = cells * steps / el
+        out["sweep_tiles"] = list(tiles)
+        out["steps"] = steps
+    out["value"] = out["member_cell_updates_per_s_graph"]
+    out["unit"] = "member cell-updates/s"
+    log(f"bench: ensemble {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +226,8 @@ def main():
                     help="'both': value is the mixed step, the FP64 step of the same state is timed beside it")
     ap.add_argument("--particles", type=int, default=1 << 22, help="tracers for the particle-update rate (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="steps of the 64-member ensemble configuration (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
     ap.add_argument("--tiles", default="auto",
                     help="sweep tile variants x,y,z (e.g. 0,0,0) or 'auto' to time them before the run")
@@ -270,6 +308,8 @@ def main():
         out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
     if particles is not None:
         out["particles"] = particles
+    if args.ensemble_steps > 0:
+        out["ensemble"] = run_ensemble(gl, args.ensemble_steps, args.warmup)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

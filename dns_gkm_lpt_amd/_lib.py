@@ -20,6 +20,8 @@ EXPORTS = [
     "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
     "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune",
+    "gkm_ensemble_create", "gkm_ensemble_destroy", "gkm_ensemble_last_error", "gkm_ensemble_member",
+    "gkm_ensemble_set_tiles", "gkm_ensemble_autotune", "gkm_ensemble_run", "gkm_ensemble_dt",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
     "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_properties", "lpt_coupled_step", "lpt_heun",
 ]
@@ -87,6 +89,17 @@ def lib():
         L.gkm_set_sweep_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
         L.gkm_get_sweep_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
         L.gkm_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int * 3), C.POINTER(C.c_double * 9)]
+        L.gkm_ensemble_create.argtypes = [C.c_int, C.POINTER(Config), C.POINTER(P)]
+        L.gkm_ensemble_destroy.argtypes = [P]
+        L.gkm_ensemble_destroy.restype = None
+        L.gkm_ensemble_last_error.argtypes = [P]
+        L.gkm_ensemble_last_error.restype = C.c_char_p
+        L.gkm_ensemble_member.argtypes = [P, C.c_int]
+        L.gkm_ensemble_member.restype = P
+        L.gkm_ensemble_set_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
+        L.gkm_ensemble_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int * 3), C.POINTER(C.c_double * 9)]
+        L.gkm_ensemble_run.argtypes = [P, C.c_int, C.c_int]
+        L.gkm_ensemble_dt.argtypes = [P, _dp]
         L.gkm_stats_compute.argtypes = [P, C.POINTER(Stats)]
         L.gkm_timing_enable.argtypes = [P, C.c_int]
         L.gkm_timing_read.argtypes = [P, C.POINTER(C.c_double * 8), C.POINTER(C.c_long * 8)]
@@ -107,15 +120,20 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def make_config(nc, params, cfl=0.5, precision=GKM_FP64, prandtl=0.70, device=0, check_state=1):
+    return Config(nc=nc, precision=precision, gam=params["gam"], rho0=params.get("rho0", 1.0),
+                  Mt=params.get("Mt", 0.3), Re=params.get("Re", 30.0), mu0=params["mu0"], p0=params["p0"],
+                  t0=params.get("t0", 1.0), cfl=cfl, prandtl=prandtl, device=device, check_state=check_state)
+
+
 class Context:
     """One resident GKM-WENO run on one GPU (wraps gkm_ctx)."""
 
     def __init__(self, nc, params, cfl=0.5, precision=GKM_FP64, prandtl=0.70, device=0, check_state=1):
         self.nc = nc
         self.nt = nc + 6
-        cfg = Config(nc=nc, precision=precision, gam=params["gam"], rho0=params.get("rho0", 1.0),
-                     Mt=params.get("Mt", 0.3), Re=params.get("Re", 30.0), mu0=params["mu0"], p0=params["p0"],
-                     t0=params.get("t0", 1.0), cfl=cfl, prandtl=prandtl, device=device, check_state=check_state)
+        self._owned = True
+        cfg = make_config(nc, params, cfl, precision, prandtl, device, check_state)
         h = C.c_void_p()
         rc = lib().gkm_create(C.byref(cfg), C.byref(h))
         self._h = h
@@ -130,10 +148,17 @@ class Context:
         if rc:
             raise GkmError(rc, lib().gkm_last_error(self._h).decode())
 
+    @classmethod
+    def _view(cls, handle, nc):
+        """A context owned by something else (an ensemble member)."""
+        c = cls.__new__(cls)
+        c.nc, c.nt, c._owned, c._h = nc, nc + 6, False, C.c_void_p(handle)
+        return c
+
     def close(self):
-        if self._h and self._h.value:
+        if self._h and self._h.value and self._owned:
             lib().gkm_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -262,3 +287,65 @@ class Context:
 
     def lpt_heun(self, dt):
         self._chk(lib().lpt_heun(self._h, dt))
+
+
+class Ensemble:
+    """Independent runs stepped together (wraps gkm_ensemble).  members[m] is a
+    Context view of member m (upload / download / stats / cfl_dt)."""
+
+    def __init__(self, ncs, params_list, cfl=0.5, precision=GKM_MIXED, prandtl=0.70, device=0, check_state=1):
+        n = len(ncs)
+        cfgs = (Config * n)(*[make_config(nc, P, cfl, precision, prandtl, device, check_state)
+                              for nc, P in zip(ncs, params_list)])
+        h = C.c_void_p()
+        rc = lib().gkm_ensemble_create(n, cfgs, C.byref(h))
+        self._h = h
+        if rc:
+            msg = lib().gkm_ensemble_last_error(h).decode() if h.value else "gkm_ensemble_create failed"
+            if h.value:
+                lib().gkm_ensemble_destroy(h)
+            self._h = C.c_void_p()
+            raise GkmError(rc, msg)
+        self.members = [Context._view(lib().gkm_ensemble_member(h, m), nc) for m, nc in enumerate(ncs)]
+
+    def _chk(self, rc):
+        if rc:
+            raise GkmError(rc, lib().gkm_ensemble_last_error(self._h).decode())
+
+    def close(self):
+        if self._h and self._h.value:
+            for m in self.members:
+                m._h = C.c_void_p()
+            lib().gkm_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_tiles(self, tiles):
+        t = (C.c_int * 3)(*tiles)
+        self._chk(lib().gkm_ensemble_set_tiles(self._h, C.byref(t)))
+
+    def autotune(self, reps=2):
+        t = (C.c_int * 3)()
+        ms = (C.c_double * 9)()
+        self._chk(lib().gkm_ensemble_autotune(self._h, reps, C.byref(t), C.byref(ms)))
+        return tuple(t), np.array(ms[:]).reshape(3, 3)
+
+    def run(self, nsteps, graph=True):
+        self._chk(lib().gkm_ensemble_run(self._h, nsteps, 1 if graph else 0))
+
+    def dt(self):
+        out = np.empty(len(self.members))
+        self._chk(lib().gkm_ensemble_dt(self._h, out))
+        return out
+

@@ -762,3 +762,301 @@ int lpt_coupled_step(gkm_ctx* c, double dt)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensemble: independent members (own grid size, gas, IC, dt) stepped together.
+// Per step one launch computes every member's CFL dt (two kernels), and one
+// launch per direction sweeps every member (ens_sweep_kernel), so a step is
+// five launches whatever the number of members; a step from each buffer
+// parity is captured once into a HIP graph and replayed.  Each member is a
+// gkm_ctx, so its state goes in and out through gkm_upload_state /
+// gkm_download_state / gkm_stats_compute.  No reference counterpart.
+// ---------------------------------------------------------------------------
+struct gkm_ensemble {
+    int n = 0;
+    bool mixed = false;
+    std::vector<gkm_ctx*> mem;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int tile[3] = {0, 0, 0};
+    gkm::SweepArgs* dargs = nullptr;  // [parity][direction][member]
+    int* dfirst = nullptr;            // [direction][member + 1] first block of each member
+    int nblk[3] = {0, 0, 0};
+    gkm::CflArgs* dcfl = nullptr;     // [parity][member]
+    double* partial = nullptr;        // n * kEnsCflBlocks
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::string err;
+};
+
+namespace {
+
+int efail(gkm_ensemble* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+#define EHIPCHK(ens, call)                                                                           \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return efail((ens), GKM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+void drop_graphs(gkm_ensemble* e)
+{
+    for (auto& g : e->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// sweep arguments of member m for parity p (state in W[p], next state W[1-p]), direction d, tile v
+gkm::SweepArgs ens_sweep_args(const gkm_ctx* c, int p, int d, int v)
+{
+    gkm::SweepArgs a{};
+    a.W = c->W[p];
+    a.dW = c->dW;
+    a.Wout = c->W[1 - p];
+    a.dt = c->dscal;
+    a.plane = c->plane;
+    a.nc = c->nc;
+    a.dx = c->dx;
+    a.inv_dx = 1.0 / c->dx;
+    a.g = c->gas;
+    a.tile = v;
+    a.seg = gkm::sweep_seg_len(d, c->nc, v);
+    a.nseg = c->nc / a.seg;
+    a.mode = d;
+    return a;
+}
+
+// upload the argument tables for tiles t[3]
+int ens_build(gkm_ensemble* e, const int t[3])
+{
+    const int n = e->n;
+    std::vector<gkm::SweepArgs> args((size_t)6 * n);
+    std::vector<int> first((size_t)3 * (n + 1));
+    for (int d = 0; d < 3; d++) {
+        int acc = 0;
+        for (int m = 0; m < n; m++) {
+            first[(size_t)d * (n + 1) + m] = acc;
+            for (int p = 0; p < 2; p++) args[((size_t)p * 3 + d) * n + m] = ens_sweep_args(e->mem[m], p, d, t[d]);
+            const gkm::SweepArgs& a = args[(size_t)d * n + m];
+            acc += gkm::sweep_blocks(d, a.nc, t[d], a.seg);
+        }
+        first[(size_t)d * (n + 1) + n] = acc;
+        e->nblk[d] = acc;
+    }
+    std::vector<gkm::CflArgs> cfl((size_t)2 * n);
+    for (int p = 0; p < 2; p++)
+        for (int m = 0; m < n; m++) {
+            const gkm_ctx* c = e->mem[m];
+            cfl[(size_t)p * n + m] = gkm::CflArgs{c->W[p], c->plane, c->gas.gam, c->cfg.cfl, c->dx, c->dscal,
+                                                  c->dscal + 1, c->dflag};
+        }
+    EHIPCHK(e, hipMemcpyAsync(e->dargs, args.data(), sizeof(gkm::SweepArgs) * args.size(), hipMemcpyHostToDevice,
+                              e->stream));
+    EHIPCHK(e, hipMemcpyAsync(e->dfirst, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, e->stream));
+    EHIPCHK(e, hipMemcpyAsync(e->dcfl, cfl.data(), sizeof(gkm::CflArgs) * cfl.size(), hipMemcpyHostToDevice,
+                              e->stream));
+    EHIPCHK(e, hipStreamSynchronize(e->stream));  // the host vectors go out of scope
+    for (int d = 0; d < 3; d++) e->tile[d] = t[d];
+    drop_graphs(e);
+    return GKM_OK;
+}
+
+// one step of every member from parity p: CFL dt, then the x, y, z sweeps
+hipError_t ens_step(gkm_ensemble* e, int p)
+{
+    const int n = e->n;
+    hipError_t r = gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream);
+    for (int d = 0; d < 3 && r == hipSuccess; d++)
+        r = gkm::launch_ens_sweep(d, e->tile[d], e->dargs + ((size_t)p * 3 + d) * n, e->dfirst + (size_t)d * (n + 1),
+                                  n, e->nblk[d], e->mixed, e->stream);
+    return r;
+}
+
+int ens_parity(gkm_ensemble* e, int* p)
+{
+    for (int m = 0; m < e->n; m++) {
+        if (!e->mem[m]->have_state) return efail(e, GKM_ENOTREADY, "member " + std::to_string(m) + " has no state");
+        if (e->mem[m]->cur != e->mem[0]->cur)
+            return efail(e, GKM_EINVAL, "members were stepped separately; their buffers are out of phase");
+    }
+    *p = e->mem[0]->cur;
+    return GKM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gkm_ensemble_create(int n, const gkm_config* cfgs, gkm_ensemble** out)
+{
+    if (!out) return GKM_EINVAL;
+    *out = nullptr;
+    if (n < 1 || n > 4096 || !cfgs) return GKM_EINVAL;
+    auto* e = new (std::nothrow) gkm_ensemble;
+    if (!e) return GKM_ENOMEM;
+    *out = e;
+    e->n = n;
+    e->mixed = cfgs[0].precision == GKM_MIXED;
+    for (int m = 0; m < n; m++) {
+        if (cfgs[m].precision != cfgs[0].precision || cfgs[m].device != cfgs[0].device)
+            return efail(e, GKM_EINVAL, "members must share precision and device");
+        gkm_ctx* c = nullptr;
+        const int rc = gkm_create(&cfgs[m], &c);
+        if (rc) {
+            const std::string msg = c ? c->err : "gkm_create failed";
+            gkm_destroy(c);
+            return efail(e, rc, "member " + std::to_string(m) + ": " + msg);
+        }
+        e->mem.push_back(c);
+    }
+    EHIPCHK(e, hipSetDevice(cfgs[0].device));
+    EHIPCHK(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    EHIPCHK(e, hipEventCreate(&e->ev[0]));
+    EHIPCHK(e, hipEventCreate(&e->ev[1]));
+    if (hipMalloc(&e->dargs, sizeof(gkm::SweepArgs) * 6 * (size_t)n) != hipSuccess ||
+        hipMalloc(&e->dfirst, sizeof(int) * 3 * (size_t)(n + 1)) != hipSuccess ||
+        hipMalloc(&e->dcfl, sizeof(gkm::CflArgs) * 2 * (size_t)n) != hipSuccess ||
+        hipMalloc(&e->partial, sizeof(double) * gkm::kEnsCflBlocks * (size_t)n) != hipSuccess)
+        return efail(e, GKM_ENOMEM, "hipMalloc ensemble tables");
+    const int t0[3] = {0, 0, 0};
+    return ens_build(e, t0);
+}
+
+void gkm_ensemble_destroy(gkm_ensemble* e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    for (gkm_ctx* c : e->mem) gkm_destroy(c);
+    (void)hipFree(e->dargs);
+    (void)hipFree(e->dfirst);
+    (void)hipFree(e->dcfl);
+    (void)hipFree(e->partial);
+    if (e->ev[0]) (void)hipEventDestroy(e->ev[0]);
+    if (e->ev[1]) (void)hipEventDestroy(e->ev[1]);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char* gkm_ensemble_last_error(const gkm_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+gkm_ctx* gkm_ensemble_member(gkm_ensemble* e, int m)
+{
+    if (!e || m < 0 || m >= e->n) return nullptr;
+    return e->mem[m];
+}
+
+int gkm_ensemble_set_tiles(gkm_ensemble* e, const int tile[3])
+{
+    if (!e || !tile) return efail(e, GKM_EINVAL, "null argument");
+    for (int d = 0; d < 3; d++)
+        if (tile[d] < 0 || tile[d] >= gkm::kTileVariants) return efail(e, GKM_EINVAL, "tile variant out of range");
+    return ens_build(e, tile);
+}
+
+int gkm_ensemble_autotune(gkm_ensemble* e, int reps, int tile[3], double ms[9])
+{
+    if (!e || reps < 1) return efail(e, GKM_EINVAL, "bad argument");
+    int p = 0;
+    int rc = ens_parity(e, &p);
+    if (rc) return rc;
+    const int n = e->n;
+    EHIPCHK(e, gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream));  // every member's dt
+    int best[3];
+    for (int d = 0; d < 3; d++) {
+        double best_ms = 0.0;
+        best[d] = e->tile[d];
+        for (int v = 0; v < gkm::kTileVariants; v++) {
+            int t[3] = {e->tile[0], e->tile[1], e->tile[2]};
+            t[d] = v;
+            rc = ens_build(e, t);
+            if (rc) return rc;
+            // dW is scratch and Wout the next state's buffer: the members' states are not changed
+            auto launch = [&] {
+                return gkm::launch_ens_sweep(d, v, e->dargs + ((size_t)p * 3 + d) * n, e->dfirst + (size_t)d * (n + 1),
+                                             n, e->nblk[d], e->mixed, e->stream);
+            };
+            EHIPCHK(e, launch());
+            EHIPCHK(e, hipEventRecord(e->ev[0], e->stream));
+            for (int r = 0; r < reps; r++) EHIPCHK(e, launch());
+            EHIPCHK(e, hipEventRecord(e->ev[1], e->stream));
+            EHIPCHK(e, hipEventSynchronize(e->ev[1]));
+            float el = 0.f;
+            EHIPCHK(e, hipEventElapsedTime(&el, e->ev[0], e->ev[1]));
+            const double avg = el / reps;
+            if (ms) ms[3 * d + v] = avg;
+            if (v == 0 || avg < best_ms) {
+                best_ms = avg;
+                best[d] = v;
+            }
+        }
+        int t[3] = {e->tile[0], e->tile[1], e->tile[2]};
+        t[d] = best[d];
+        rc = ens_build(e, t);
+        if (rc) return rc;
+    }
+    if (tile)
+        for (int d = 0; d < 3; d++) tile[d] = e->tile[d];
+    return GKM_OK;
+}
+
+int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
+{
+    if (!e || nsteps < 0) return efail(e, GKM_EINVAL, "bad argument");
+    int p = 0;
+    int rc = ens_parity(e, &p);
+    if (rc) return rc;
+    if (use_graph) {
+        for (int q = 0; q < 2; q++) {
+            if (e->graph[q]) continue;
+            hipGraph_t g = nullptr;
+            EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t r = ens_step(e, q);
+            const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+            if (r != hipSuccess || r2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return efail(e, GKM_EHIP, std::string("graph capture: ") + hipGetErrorString(r != hipSuccess ? r : r2));
+            }
+            const hipError_t r3 = hipGraphInstantiate(&e->graph[q], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (r3 != hipSuccess) {
+                e->graph[q] = nullptr;
+                return efail(e, GKM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r3));
+            }
+        }
+    }
+    for (int s = 0; s < nsteps; s++) {
+        if (use_graph)
+            EHIPCHK(e, hipGraphLaunch(e->graph[p], e->stream));
+        else
+            EHIPCHK(e, ens_step(e, p));
+        p = 1 - p;
+    }
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    std::vector<int> flags(e->n, 0);
+    for (int m = 0; m < e->n; m++) {
+        gkm_ctx* c = e->mem[m];
+        c->cur = p;
+        c->lpt.valid = false;
+        EHIPCHK(e, hipMemcpy(&flags[m], c->dflag, sizeof(int), hipMemcpyDeviceToHost));
+    }
+    for (int m = 0; m < e->n; m++)
+        if (flags[m] && e->mem[m]->cfg.check_state)
+            return efail(e, GKM_ESTATE, "member " + std::to_string(m) + ": non-physical state (rho <= 0, p <= 0 or NaN)");
+    return GKM_OK;
+}
+
+int gkm_ensemble_dt(gkm_ensemble* e, double* dt)
+{
+    if (!e || !dt) return efail(e, GKM_EINVAL, "null argument");
+    for (int m = 0; m < e->n; m++) EHIPCHK(e, hipMemcpy(dt + m, e->mem[m]->dscal, sizeof(double), hipMemcpyDeviceToHost));
+    return GKM_OK;
+}
+
+}  // extern "C"

@@ -108,8 +108,11 @@ struct SweepIn {
 // With one wave per SIMD the compiler (ROCm 7.2, gfx950) spilled into AGPRs and
 // the x and y instances produced wrong flux differences on the GPU (the z
 // instance and every instance built this way match the FP64 oracle to 1e-14).
+//
+// sweep_body is the work of one workgroup: block `bid` of the `nblk` blocks of
+// one grid (member).  remap = true applies the XCD-aware order to bid.
 template <int DIR, bool MIX, int V>
-__global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
+__device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk, bool remap)
 {
     constexpr int CN = Tile<DIR, V>::CN, CA = Tile<DIR, V>::CA, CB = Tile<DIR, V>::CB;
     constexpr int NT = CN * CA * CB;
@@ -125,9 +128,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
 
     // workgroup -> (pencil, segment); XCD-aware: the hardware deals workgroups
     // round-robin over the 8 XCDs, so give each XCD a contiguous range of pencils.
-    const int nblk = gridDim.x;
-    int bid = blockIdx.x;
-    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+    if (remap && (nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
     const int s = bid % p.nseg;
     const int pencil = bid / p.nseg;
     const int npb = nc / CB;
@@ -241,6 +242,32 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
     }
 }
 
+template <int DIR, bool MIX, int V>
+__global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
+{
+    sweep_body<DIR, MIX, V>(p, blockIdx.x, gridDim.x, true);
+}
+
+// One launch for the sweeps of every member of an ensemble: member m owns the
+// blocks [first[m], first[m+1]) of the grid (after the XCD-aware reorder of
+// the whole grid, so each XCD gets a contiguous range of members' pencils).
+template <int DIR, bool MIX, int V>
+__global__ __launch_bounds__(256, 2) void ens_sweep_kernel(const SweepArgs* __restrict__ members,
+                                                          const int* __restrict__ first, int nmem)
+{
+    const int nblk = gridDim.x;
+    int g = blockIdx.x;
+    if ((nblk & 7) == 0) g = (g & 7) * (nblk >> 3) + (g >> 3);
+    int lo = 0, hi = nmem;  // first[lo] <= g < first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const SweepArgs p = members[lo];
+    sweep_body<DIR, MIX, V>(p, g - first[lo], first[lo + 1] - first[lo], false);
+}
+
 bool sweep_grid_ok(int nc)
 {
     // multiples of 16 fit every tile shape; nc >= 32 keeps every wrapped index
@@ -296,6 +323,46 @@ hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
+int sweep_blocks(int dir, int nc, int tile, int seg)
+{
+    if (dir < 0 || dir > 2 || tile < 0 || tile >= kTileVariants || seg <= 0) return 0;
+    const int CA = kTiles[dir][tile][1], CB = kTiles[dir][tile][2];
+    return (nc / CA) * (nc / CB) * (nc / seg);
+}
+
+template <int DIR, int V>
+static hipError_t launch_ens_tile(const SweepArgs* members, const int* first, int nmem, int nblk, bool mixed,
+                                  hipStream_t s)
+{
+    if (mixed)
+        hipLaunchKernelGGL((ens_sweep_kernel<DIR, true, V>), dim3((unsigned)nblk), dim3(256), 0, s, members, first, nmem);
+    else
+        hipLaunchKernelGGL((ens_sweep_kernel<DIR, false, V>), dim3((unsigned)nblk), dim3(256), 0, s, members, first, nmem);
+    return hipGetLastError();
+}
+
+template <int DIR>
+static hipError_t launch_ens_dir(int tile, const SweepArgs* members, const int* first, int nmem, int nblk, bool mixed,
+                                 hipStream_t s)
+{
+    switch (tile) {
+    case 0: return launch_ens_tile<DIR, 0>(members, first, nmem, nblk, mixed, s);
+    case 1: return launch_ens_tile<DIR, 1>(members, first, nmem, nblk, mixed, s);
+    case 2: return launch_ens_tile<DIR, 2>(members, first, nmem, nblk, mixed, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ens_sweep(int dir, int tile, const SweepArgs* members, const int* first, int nmem, int nblk,
+                            bool mixed, hipStream_t s)
+{
+    if (nmem <= 0 || nblk <= 0) return hipErrorInvalidValue;
+    if (dir == 0) return launch_ens_dir<0>(tile, members, first, nmem, nblk, mixed, s);
+    if (dir == 1) return launch_ens_dir<1>(tile, members, first, nmem, nblk, mixed, s);
+    if (dir == 2) return launch_ens_dir<2>(tile, members, first, nmem, nblk, mixed, s);
+    return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------
 // CFL: dt = cfl dx / max(|U| + sqrt(gam p / rho)) over all cells
 // (timestep, update.cpp:1031-1054).  A block maximum goes to partial[]; a
@@ -337,8 +404,9 @@ __global__ __launch_bounds__(256) void cfl_partial_kernel(const double* __restri
     }
 }
 
-__global__ __launch_bounds__(256) void cfl_final_kernel(const double* partial, int n, double cfl, double dx,
-                                                        double* dt, double* vmax, int* bad)
+// block reduction of n partial maxima (-1 = non-physical state seen) -> dt
+__device__ __forceinline__ void cfl_reduce(const double* partial, int n, double cfl, double dx, double* dt, double* vmax,
+                                           int* bad)
 {
     double m = 0.0;
     int b = 0;
@@ -366,6 +434,12 @@ __global__ __launch_bounds__(256) void cfl_final_kernel(const double* partial, i
     }
 }
 
+__global__ __launch_bounds__(256) void cfl_final_kernel(const double* partial, int n, double cfl, double dx,
+                                                        double* dt, double* vmax, int* bad)
+{
+    cfl_reduce(partial, n, cfl, dx, dt, vmax, bad);
+}
+
 hipError_t launch_cfl(const double* W, long plane, double gam, double cfl, double dx, double* partial, double* dt,
                       double* vmax, int* bad, hipStream_t s)
 {
@@ -373,6 +447,58 @@ hipError_t launch_cfl(const double* W, long plane, double gam, double cfl, doubl
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(cfl_final_kernel, dim3(1), dim3(256), 0, s, partial, kCflBlocks, cfl, dx, dt, vmax, bad);
+    return hipGetLastError();
+}
+
+// Ensemble CFL: kEnsCflBlocks blocks per member reduce max(|U|+c) of that
+// member's cells into partial[m * kEnsCflBlocks + b]; one block per member then
+// writes dt = cfl dx / max (0 and the flag when a state is non-physical).
+__global__ __launch_bounds__(256) void ens_cfl_partial_kernel(const CflArgs* __restrict__ members, double* partial)
+{
+    const int m = blockIdx.x / kEnsCflBlocks, b = blockIdx.x % kEnsCflBlocks;
+    const CflArgs a = members[m];
+    double mx = 0.0;
+    bool bad = false;
+    for (long c = (long)b * blockDim.x + threadIdx.x; c < a.plane; c += (long)kEnsCflBlocks * blockDim.x) {
+        const double* W = a.W;
+        const long plane = a.plane;
+        const double den = W[c];
+        const double ir = 1.0 / den;
+        const double u = W[plane + c] * ir, v = W[2 * plane + c] * ir, w = W[3 * plane + c] * ir;
+        const double Vsqr = u * u + v * v + w * w;
+        const double P = (den * (a.gam - 1)) * (W[4 * plane + c] * ir - 0.5 * Vsqr);
+        const double val = sqrt(Vsqr) + sqrt(a.gam * P * ir);
+        if (!(den > 0.0) || !(P > 0.0) || !isfinite(val)) bad = true;
+        else mx = fmax(mx, val);
+    }
+    __shared__ double sm[4];
+    __shared__ int sbad;
+    if (threadIdx.x == 0) sbad = 0;
+    __syncthreads();
+    if (bad) sbad = 1;
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double r = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+        partial[blockIdx.x] = sbad ? -1.0 : r;
+    }
+}
+
+__global__ __launch_bounds__(256) void ens_cfl_final_kernel(const CflArgs* __restrict__ members, const double* partial)
+{
+    const CflArgs a = members[blockIdx.x];
+    cfl_reduce(partial + (long)blockIdx.x * kEnsCflBlocks, kEnsCflBlocks, a.cfl, a.dx, a.dt, a.vmax, a.bad);
+}
+
+hipError_t launch_ens_cfl(const CflArgs* members, int nmem, double* partial, hipStream_t s)
+{
+    if (nmem <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ens_cfl_partial_kernel, dim3((unsigned)(nmem * kEnsCflBlocks)), dim3(256), 0, s, members,
+                       partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ens_cfl_final_kernel, dim3((unsigned)nmem), dim3(256), 0, s, members, partial);
     return hipGetLastError();
 }
 

@@ -40,6 +40,12 @@ struct Tile {
 };
 
 hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s);
+// blocks of one grid's sweep launch for a tile variant and segment length
+int sweep_blocks(int dir, int nc, int tile, int seg);
+// one launch sweeping every member of an ensemble: members[m] (device array) with
+// blocks [first[m], first[m+1]) of nblk = first[nmem]; one tile variant for all
+hipError_t launch_ens_sweep(int dir, int tile, const SweepArgs* members, const int* first, int nmem, int nblk,
+                            bool mixed, hipStream_t s);
 int sweep_seg_len(int dir, int nc, int tile);  // faces per workgroup chosen for a grid
 bool sweep_grid_ok(int nc);                    // nc compatible with every tile shape
 
@@ -48,6 +54,16 @@ bool sweep_grid_ok(int nc);                    // nc compatible with every tile 
 constexpr int kCflBlocks = 1024;
 hipError_t launch_cfl(const double* W, long plane, double gam, double cfl, double dx, double* partial, double* dt,
                       double* vmax, int* bad, hipStream_t s);
+// ensemble CFL: one dt per member (partial holds nmem * kEnsCflBlocks doubles)
+struct CflArgs {
+    const double* W;
+    long plane;
+    double gam, cfl, dx;
+    double *dt, *vmax;
+    int* bad;
+};
+constexpr int kEnsCflBlocks = 64;
+hipError_t launch_ens_cfl(const CflArgs* members, int nmem, double* partial, hipStream_t s);
 
 // padded (nt^3 per plane, ghosts) <-> unpadded (nc^3 per plane)
 hipError_t launch_unpack(const double* padded, double* W, int nc, hipStream_t s);

@@ -135,6 +135,31 @@ int gkm_flux_batch(gkm_ctx *ctx, long n, const double *in, const double *tau, do
                    int precision, double *F);
 int gkm_weno_batch(gkm_ctx *ctx, long n, const double *in, int precision, double *out);
 
+/* ---- Ensemble of independent runs (no reference counterpart) -------------
+ * n members, each a full context (own grid size, gas, initial condition,
+ * CFL dt), stepped together: per step one launch computes every member's dt
+ * and one launch per direction sweeps every member; with use_graph the step
+ * is a captured HIP graph (one per buffer parity) replayed nsteps times.
+ * Members must share precision and device.  A member's state goes in and out
+ * through gkm_upload_state / gkm_download_state / gkm_stats_compute on
+ * gkm_ensemble_member(e, m) (owned by the ensemble; do not destroy it, and do
+ * not step a member on its own between ensemble runs).  Each member's result
+ * is its single run's to rounding. */
+typedef struct gkm_ensemble gkm_ensemble;
+int gkm_ensemble_create(int n, const gkm_config *cfgs, gkm_ensemble **out);
+void gkm_ensemble_destroy(gkm_ensemble *e);
+const char *gkm_ensemble_last_error(const gkm_ensemble *e);
+gkm_ctx *gkm_ensemble_member(gkm_ensemble *e, int m);
+/* one tile variant per direction for all members; gkm_ensemble_autotune times
+ * every variant of every direction on the members' current states (which it
+ * does not change) and keeps the fastest; ms (may be NULL) as gkm_autotune */
+int gkm_ensemble_set_tiles(gkm_ensemble *e, const int tile[3]);
+int gkm_ensemble_autotune(gkm_ensemble *e, int reps, int tile[3], double ms[9]);
+/* nsteps steps of every member, each with dt from its own CFL condition */
+int gkm_ensemble_run(gkm_ensemble *e, int nsteps, int use_graph);
+/* the dt of each member's last step (n doubles) */
+int gkm_ensemble_dt(gkm_ensemble *e, double *dt);
+
 /* ---- Lagrangian particle tracker (ParticleTrack.cpp, update.cpp:1056-1247) */
 /* Upload np positions (np x 3, order y, x, z); replaces locateParticles
  * (update.cpp:1286-1322). Tags are 0..np-1 in upload order. */

@@ -20,6 +20,7 @@ EXPORTS = [
     "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
     "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune",
+    "gkm_tl_run", "gkm_adjoint_run",
     "gkm_ensemble_create", "gkm_ensemble_destroy", "gkm_ensemble_last_error", "gkm_ensemble_member",
     "gkm_ensemble_set_tiles", "gkm_ensemble_autotune", "gkm_ensemble_run", "gkm_ensemble_dt",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
@@ -89,6 +90,8 @@ def lib():
         L.gkm_set_sweep_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
         L.gkm_get_sweep_tiles.argtypes = [P, C.POINTER(C.c_int * 3)]
         L.gkm_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int * 3), C.POINTER(C.c_double * 9)]
+        L.gkm_tl_run.argtypes = [P, C.c_double, C.c_int, _dp, _dp]
+        L.gkm_adjoint_run.argtypes = [P, C.c_double, C.c_int, C.c_int, _dp, _dp]
         L.gkm_ensemble_create.argtypes = [C.c_int, C.POINTER(Config), C.POINTER(P)]
         L.gkm_ensemble_destroy.argtypes = [P]
         L.gkm_ensemble_destroy.restype = None
@@ -207,6 +210,22 @@ class Context:
 
     def sync(self):
         self._chk(lib().gkm_sync(self._h))
+
+    def tl_run(self, dt, nsteps, dW0):
+        """Tangent-linear run from the resident state (advanced by nsteps):
+        returns dW_N for the perturbation dW0 (padded layout)."""
+        dW0 = np.ascontiguousarray(dW0, dtype=np.float64)
+        out = np.empty((5, self.nt, self.nt, self.nt))
+        self._chk(lib().gkm_tl_run(self._h, dt, nsteps, dW0, out))
+        return out
+
+    def adjoint_run(self, dt, nsteps, every, lamN):
+        """Adjoint run from the resident state (advanced by nsteps): returns
+        lam_0 = (dW_N/dW_0)^T lam_N (padded layout)."""
+        lamN = np.ascontiguousarray(lamN, dtype=np.float64)
+        out = np.empty((5, self.nt, self.nt, self.nt))
+        self._chk(lib().gkm_adjoint_run(self._h, dt, nsteps, every, lamN, out))
+        return out
 
     def set_tiles(self, tiles):
         t = (C.c_int * 3)(*tiles)

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/dnsgkm.h"
+#include "adj_kernels.hpp"
 #include "gkm_kernels.hpp"
 #include "lpt_kernels.hpp"
 
@@ -43,6 +44,10 @@ struct gkm_ctx {
     long tcount[8] = {0};
     // particles
     lpt::State lpt;
+    // tangent-linear / adjoint (allocated on first use)
+    double* adj_scratch = nullptr;
+    double* adj_vec[2] = {nullptr, nullptr};  // tangent or adjoint vectors, 5 x plane each
+    std::vector<double*> adj_states;          // checkpoints and segment states
 };
 
 namespace {
@@ -177,6 +182,10 @@ void gkm_destroy(gkm_ctx* c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     lpt::release(c->lpt);
+    (void)hipFree(c->adj_scratch);
+    (void)hipFree(c->adj_vec[0]);
+    (void)hipFree(c->adj_vec[1]);
+    for (double* p : c->adj_states) (void)hipFree(p);
     hipFree(c->W[0]);
     hipFree(c->W[1]);
     hipFree(c->dW);
@@ -297,6 +306,108 @@ int gkm_sweep_flux_diff(gkm_ctx* c, int dir, double dt, double* out)
     HIPCHK(c, hipMemcpyAsync(out, c->dW, sizeof(double) * 5 * c->plane, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GKM_OK;
+}
+
+// ---- tangent-linear and adjoint of the fixed-dt FP64 step (no reference counterpart)
+namespace {
+int adj_prepare(gkm_ctx* c, double dt, int nstates)
+{
+    if (c->cfg.precision != GKM_FP64) return fail(c, GKM_EINVAL, "tangent-linear / adjoint runs need GKM_FP64");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, GKM_EINVAL, "dt must be positive and finite");
+    const size_t sb = sizeof(double) * 5 * (size_t)c->plane;
+    if (!c->adj_scratch &&
+        hipMalloc(&c->adj_scratch, sizeof(double) * gkm::adj_scratch_doubles(c->plane)) != hipSuccess)
+        return fail(c, GKM_ENOMEM, "hipMalloc adjoint scratch");
+    for (auto& v : c->adj_vec)
+        if (!v && hipMalloc(&v, sb) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc adjoint vectors");
+    while ((int)c->adj_states.size() < nstates) {
+        double* p = nullptr;
+        if (hipMalloc(&p, sb) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc checkpoints");
+        c->adj_states.push_back(p);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dscal, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // dt lives on the host stack
+    return GKM_OK;
+}
+
+int vec_upload(gkm_ctx* c, const double* host_padded, double* dev)
+{
+    HIPCHK(c, hipMemcpyAsync(c->stage, host_padded, sizeof(double) * 5 * c->padded, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, gkm::launch_unpack(c->stage, dev, c->nc, c->stream));
+    return GKM_OK;
+}
+
+int vec_download(gkm_ctx* c, const double* dev, double* host_padded)
+{
+    HIPCHK(c, gkm::launch_pack(dev, c->stage, c->nc, c->stream));
+    HIPCHK(c, hipMemcpyAsync(host_padded, c->stage, sizeof(double) * 5 * c->padded, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+}  // namespace
+
+int gkm_tl_run(gkm_ctx* c, double dt, int nsteps, const double* dW0, double* dWN)
+{
+    if (!c || !dW0 || !dWN || nsteps < 0) return fail(c, GKM_EINVAL, "bad argument");
+    int rc = adj_prepare(c, dt, 0);
+    if (rc) return rc;
+    gkm::AdjScratch s{};
+    gkm::adj_scratch_carve(c->adj_scratch, c->plane, s);
+    rc = vec_upload(c, dW0, c->adj_vec[0]);
+    if (rc) return rc;
+    int t = 0;
+    for (int n = 0; n < nsteps; n++) {
+        HIPCHK(c, gkm::launch_tl_step(c->W[c->cur], c->adj_vec[t], c->plane, c->nc, c->gas, dt, c->dx, s,
+                                      c->adj_vec[1 - t], c->stream));
+        t = 1 - t;
+        rc = eulerian_step(c);
+        if (rc) return rc;
+    }
+    return vec_download(c, c->adj_vec[t], dWN);
+}
+
+int gkm_adjoint_run(gkm_ctx* c, double dt, int nsteps, int every, const double* lamN, double* lam0)
+{
+    if (!c || !lamN || !lam0 || nsteps < 0 || every < 1) return fail(c, GKM_EINVAL, "bad argument");
+    const int nseg = (nsteps + every - 1) / every;
+    // checkpoints, the states of one segment, and the final state
+    int rc = adj_prepare(c, dt, nseg + every + 1);
+    if (rc) return rc;
+    gkm::AdjScratch s{};
+    gkm::adj_scratch_carve(c->adj_scratch, c->plane, s);
+    const size_t sb = sizeof(double) * 5 * (size_t)c->plane;
+    double** ckpt = c->adj_states.data();
+    double** seg = ckpt + nseg;
+    double* final_state = ckpt[nseg + every];
+    for (int n = 0; n < nsteps; n++) {
+        if (n % every == 0) HIPCHK(c, hipMemcpyAsync(ckpt[n / every], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+        rc = eulerian_step(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(final_state, c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+    rc = vec_upload(c, lamN, c->adj_vec[0]);
+    if (rc) return rc;
+    int l = 0;
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int s0 = k * every, s1 = (s0 + every < nsteps) ? s0 + every : nsteps;
+        HIPCHK(c, hipMemcpyAsync(c->W[c->cur], ckpt[k], sb, hipMemcpyDeviceToDevice, c->stream));
+        for (int n = s0; n < s1; n++) {
+            HIPCHK(c, hipMemcpyAsync(seg[n - s0], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+            if (n + 1 < s1) {
+                rc = eulerian_step(c);
+                if (rc) return rc;
+            }
+        }
+        for (int n = s1 - 1; n >= s0; n--) {
+            HIPCHK(c, gkm::launch_adj_step(seg[n - s0], c->adj_vec[l], c->plane, c->nc, c->gas, dt, c->dx, s,
+                                           c->adj_vec[1 - l], c->stream));
+            l = 1 - l;
+        }
+    }
+    HIPCHK(c, hipMemcpyAsync(c->W[c->cur], final_state, sb, hipMemcpyDeviceToDevice, c->stream));
+    c->lpt.valid = false;
+    return vec_download(c, c->adj_vec[l], lam0);
 }
 
 int gkm_set_sweep_tiles(gkm_ctx* c, const int tile[3])

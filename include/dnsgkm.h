@@ -135,6 +135,19 @@ int gkm_flux_batch(gkm_ctx *ctx, long n, const double *in, const double *tau, do
                    int precision, double *F);
 int gkm_weno_batch(gkm_ctx *ctx, long n, const double *in, int precision, double *out);
 
+/* ---- Tangent-linear and discrete adjoint of the fixed-dt FP64 step --------
+ * (no reference counterpart).  Both start from the resident state W_0, use the
+ * fixed dt for nsteps steps of the same FP64 step gkm_step evaluates, and
+ * leave the resident state at W_N.  Vectors are in the padded host layout
+ * (5 x (nc+6)^3, like gkm_upload_state_packed): input ghost layers are
+ * ignored, output ghost layers are filled periodically.
+ * gkm_tl_run: dW_N = (dW_N/dW_0) dW_0.
+ * gkm_adjoint_run: lam_0 = (dW_N/dW_0)^T lam_N, with the forward trajectory
+ * checkpointed every `every` steps and each segment recomputed on the way back.
+ * Contexts must be GKM_FP64. */
+int gkm_tl_run(gkm_ctx *ctx, double dt, int nsteps, const double *dW0, double *dWN);
+int gkm_adjoint_run(gkm_ctx *ctx, double dt, int nsteps, int every, const double *lamN, double *lam0);
+
 /* ---- Ensemble of independent runs (no reference counterpart) -------------
  * n members, each a full context (own grid size, gas, initial condition,
  * CFL dt), stepped together: per step one launch computes every member's dt

@@ -18,13 +18,18 @@ from . import _lib as gl
 from . import ic as icm
 
 
-def kinetic_energy(W):
-    """Volume-averaged (1/2) rho |u|^2 of a padded state."""
+def cell_kinetic_energy(W):
+    """(1/2) rho |u|^2 of each interior cell of a padded state."""
     nc = W.shape[-1] - 6
     inner = (slice(3, nc + 3),) * 3
     rho = W[0][inner]
     m2 = W[1][inner] ** 2 + W[2][inner] ** 2 + W[3][inner] ** 2
-    return float(np.sum(0.5 * m2 / rho) / nc ** 3)
+    return 0.5 * m2 / rho
+
+
+def kinetic_energy(W):
+    """Volume-averaged (1/2) rho |u|^2 of a padded state."""
+    return float(np.mean(cell_kinetic_energy(W)))
 
 
 def kinetic_energy_adjoint(W):
@@ -66,13 +71,24 @@ class Problem:
     def context(self):
         return gl.Context(self.nc, self.params, precision=gl.GKM_FP64)
 
-    def objective(self, u):
-        """J(u) by a forward run."""
+    def final_state(self, u):
+        """W_N of a forward run from u."""
         with self.context() as c:
             c.upload(initial_state(u, self.rho, self.p, self.params["gam"]))
             for _ in range(self.nsteps):
                 c.step(self.dt)
-            return kinetic_energy(c.download())
+            return c.download()
+
+    def objective(self, u):
+        """J(u) by a forward run."""
+        return kinetic_energy(self.final_state(u))
+
+    def central_difference(self, u, d, h):
+        """(J(u + h d) - J(u - h d)) / 2h, summed as per-cell differences so
+        that the rounding of J itself (an ulp of J, ~1e-16 J) does not enter."""
+        ep = cell_kinetic_energy(self.final_state(u + h * d))
+        em = cell_kinetic_energy(self.final_state(u - h * d))
+        return float(np.mean(ep - em)) / (2 * h)
 
     def gradient(self, u):
         """(J(u), dJ/du) by a forward run and an adjoint run."""
@@ -124,3 +140,92 @@ def KineticEnergy(problem):
             return gJ * g
 
     return _KE
+
+
+def band_limited_direction(shape, seed, kmax=4):
+    """A seeded random unit vector field (shape (3, nc, nc, nc)) holding only
+    the Fourier modes |k| <= kmax (the initial condition's band)."""
+    rng = np.random.default_rng(seed)
+    nc = shape[-1]
+    k = np.fft.fftfreq(nc, 1.0 / nc)
+    kk = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2)
+    d = np.stack([np.real(np.fft.ifftn(np.fft.fftn(rng.standard_normal((nc,) * 3)) * (kk <= kmax)))
+                  for _ in range(shape[0])])
+    return d / np.linalg.norm(d)
+
+
+def white_noise_direction(shape, seed):
+    """A seeded random unit vector field with independent Gaussian cells."""
+    d = np.random.default_rng(seed).standard_normal(shape)
+    return d / np.linalg.norm(d)
+
+
+def make_problem(nc, k0=4.0, Mt=0.3, Re=40.0, seed=0, nsteps=200, every=20, cfl=0.5):
+    """The reference's initial condition (Initialize_DHIT, FlowField.cpp:89-105)
+    with dt fixed from its CFL number; returns (Problem, u0)."""
+    ic = icm.make_ic(nc, k0=k0, Mt=Mt, Re=Re, seed=seed)
+    P = ic["params"]
+    u = np.stack([ic["u"], ic["v"], ic["w"]])
+    with gl.Context(nc, P, precision=gl.GKM_FP64, cfl=cfl) as c:
+        c.upload(initial_state(u, P["rho0"], P["p0"], P["gam"]))
+        dt = c.cfl_dt()
+    return Problem(nc, P, dt, nsteps, every), u
+
+
+def check_gradient(problem, u, directions, h_rels=(1e-6,), g=None):
+    """Compare the adjoint gradient with central differences of J along each
+    unit direction d, h = h_rel |u| for every h_rel given, and with the
+    tangent-linear derivative along d.  A given gradient g is reused (J is then
+    None).  Returns (J, g, report): the gradient's wall time and, per
+    direction, the adjoint, tangent-linear and finite-difference derivatives
+    with their relative errors (keyed by h_rel) and wall times."""
+    import time
+    t0 = time.perf_counter()
+    J, g = problem.gradient(u) if g is None else (None, g)
+    t_grad = time.perf_counter() - t0
+    rows = []
+    for d in directions:
+        ad = float(np.sum(g * d))
+        t0 = time.perf_counter()
+        tl = problem.directional(u, d)
+        row = {"adjoint": ad, "tl": tl, "err_tl": abs(tl - ad) / abs(ad), "s_tl": time.perf_counter() - t0,
+               "fd": {}, "err_fd": {}}
+        for hr in h_rels:
+            fd = problem.central_difference(u, d, hr * np.linalg.norm(u))
+            row["fd"][f"{hr:g}"] = fd
+            row["err_fd"][f"{hr:g}"] = abs(fd - ad) / abs(ad)
+        rows.append(row)
+    return J, g, {"s_gradient": t_grad, "rows": rows}
+
+
+def main(argv=None):
+    """Gradient check of the 64^3 / 200-step / checkpoint-every-20 problem
+    (Mt 0.3, Re_lambda 40, dt fixed from the initial CFL 0.5): seeded
+    band-limited and white-noise unit directions, one JSON line."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--nc", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--dirs", type=int, default=8)
+    ap.add_argument("--white", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--h", default="1e-6,1e-8", help="finite-difference steps relative to |u0|")
+    a = ap.parse_args(argv)
+    hs = [float(x) for x in a.h.split(",")]
+    pb, u = make_problem(a.nc, seed=a.seed, nsteps=a.steps, every=a.every)
+    dirs = [band_limited_direction(u.shape, 100 + k) for k in range(a.dirs)]
+    J, g, rep = check_gradient(pb, u, dirs, hs)
+    wdirs = [white_noise_direction(u.shape, 200 + k) for k in range(a.white)]
+    _, _, wrep = check_gradient(pb, u, wdirs, hs, g=g)
+    out = {"nc": a.nc, "steps": a.steps, "every": a.every, "dt": pb.dt, "J": J, "grad_norm": float(np.linalg.norm(g)),
+           "s_gradient": rep["s_gradient"], "band_limited": rep["rows"], "white_noise": wrep["rows"],
+           "max_err_fd": {f"{h:g}": max(r["err_fd"][f"{h:g}"] for r in rep["rows"]) for h in hs},
+           "max_err_tl": max(r["err_tl"] for r in rep["rows"])}
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

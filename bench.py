@@ -183,9 +183,9 @@ def run_particles(ctx, npart, steps):
 def run_ensemble(gl, steps, warmup):
     """The batched-ensemble configuration: 64 independent realizations (24 at
     32^3, 24 at 48^3, 16 at 64^3), each with its own seed, k0 in [3, 6], Mt in
-    [0.1, 0.6], Re_lambda in [20, 60], mixed precision, tiles autotuned; steps
-    replayed from captured HIP graphs, timed beside the same steps launched
-    eagerly."""
+    [0.1, 0.6], Re_lambda in [20, 60], mixed precision, 8192 tracers each,
+    tiles autotuned; steps replayed from captured HIP graphs, timed beside the
+    same steps launched eagerly."""
     from dns_gkm_lpt_amd import ic as icm
 
     rng = np.random.default_rng(4)
@@ -196,10 +196,13 @@ def run_ensemble(gl, steps, warmup):
         ic = icm.make_ic(nc, k0=k0, Mt=Mt, Re=Re, seed=100 + m)
         cases.append((nc, ic["params"], icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"])))
     cells = sum(nc ** 3 for nc in ncs)
-    out = {"members": len(ncs), "grids": "24 x 32^3 + 24 x 48^3 + 16 x 64^3", "precision": "mixed"}
+    ntr = 8192
+    out = {"members": len(ncs), "grids": "24 x 32^3 + 24 x 48^3 + 16 x 64^3", "precision": "mixed",
+           "tracers_per_member": ntr}
     with gl.Ensemble(ncs, [c[1] for c in cases], cfl=0.5, precision=gl.GKM_MIXED) as e:
         for m, c in enumerate(cases):
             e.members[m].upload(c[2])
+            e.members[m].lpt_upload(icm.uniform_particles(ntr, seed=100 + m))
         tiles, _ = e.autotune(reps=2)
         e.run(warmup, graph=True)
         for mode, graph in (("graph", True), ("eager", False)):
@@ -208,6 +211,7 @@ def run_ensemble(gl, steps, warmup):
             el = time.perf_counter() - t
             out[f"ms_per_step_{mode}"] = 1e3 * el / steps
             out[f"member_cell_updates_per_s_{mode}"] = cells * steps / el
+            out[f"particle_updates_per_s_{mode}"] = ntr * len(ncs) * steps / el
         out["sweep_tiles"] = list(tiles)
         out["steps"] = steps
     out["value"] = out["member_cell_updates_per_s_graph"]
@@ -226,7 +230,7 @@ def main():
                     help="'both': value is the mixed step, the FP64 step of the same state is timed beside it")
     ap.add_argument("--particles", type=int, default=1 << 22, help="tracers for the particle-update rate (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--ensemble-steps", type=int, default=20,
+    ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 64-member ensemble configuration (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
     ap.add_argument("--tiles", default="auto",

@@ -310,7 +310,9 @@ class Context:
 
 class Ensemble:
     """Independent runs stepped together (wraps gkm_ensemble).  members[m] is a
-    Context view of member m (upload / download / stats / cfl_dt)."""
+    Context view of member m (upload / download / stats / cfl_dt, lpt_upload /
+    lpt_download / lpt_stats); tracers uploaded to a member advance with it,
+    as lpt_coupled_step advances them in a single context."""
 
     def __init__(self, ncs, params_list, cfl=0.5, precision=GKM_MIXED, prandtl=0.70, device=0, check_state=1):
         n = len(ncs)

@@ -7,6 +7,7 @@
 // last-error string and return a GKM_E* code.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -180,22 +181,22 @@ int gkm_create(const gkm_config* cfg, gkm_ctx** out)
 void gkm_destroy(gkm_ctx* c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     lpt::release(c->lpt);
     (void)hipFree(c->adj_scratch);
     (void)hipFree(c->adj_vec[0]);
     (void)hipFree(c->adj_vec[1]);
     for (double* p : c->adj_states) (void)hipFree(p);
-    hipFree(c->W[0]);
-    hipFree(c->W[1]);
-    hipFree(c->dW);
-    hipFree(c->stage);
-    hipFree(c->dscal);
-    hipFree(c->dflag);
-    hipFree(c->partial);
-    if (c->ev[0]) hipEventDestroy(c->ev[0]);
-    if (c->ev[1]) hipEventDestroy(c->ev[1]);
-    if (c->stream) hipStreamDestroy(c->stream);
+    (void)hipFree(c->W[0]);
+    (void)hipFree(c->W[1]);
+    (void)hipFree(c->dW);
+    (void)hipFree(c->stage);
+    (void)hipFree(c->dscal);
+    (void)hipFree(c->dflag);
+    (void)hipFree(c->partial);
+    if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
+    if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -215,6 +216,7 @@ int gkm_upload_state(gkm_ctx* c, const double* const W[5])
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
+    c->lpt.valid = false;  // a spline built earlier described the previous state
     return GKM_OK;
 }
 
@@ -515,9 +517,9 @@ int gkm_flux_batch(gkm_ctx* c, long n, const double* in, const double* tau, doub
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, GKM_EHIP, std::string("flux batch: ") + hipGetErrorString(e));
     }
-    hipFree(din);
-    hipFree(dtau);
-    hipFree(dF);
+    (void)hipFree(din);
+    (void)hipFree(dtau);
+    (void)hipFree(dF);
     return rc;
 }
 
@@ -537,8 +539,8 @@ int gkm_weno_batch(gkm_ctx* c, long n, const double* in, int precision, double* 
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, GKM_EHIP, std::string("weno batch: ") + hipGetErrorString(e));
     }
-    hipFree(din);
-    hipFree(dout);
+    (void)hipFree(din);
+    (void)hipFree(dout);
     return rc;
 }
 
@@ -593,13 +595,10 @@ int build_spline(gkm_ctx* c, int b)
     int rc = ensure_spline_mem(c);
     if (rc) return rc;
     const lpt::Geom g = geom_of(c);
+    if (!lpt::ens_z_chunks(g.nt)) return fail(c, GKM_EINVAL, "splines need nc <= 1082");
     return timed(c, 4, [&] {
-        for (int q = 1; q <= 3; q++) {
-            hipError_t e = lpt::launch_prefilter(c->W[c->cur], c->plane, q, c->gas.gam, g, c->lpt.pivots, c->lpt.coef[b][q - 1],
-                                                 c->stream);
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
+        return lpt::launch_prefilter_velocity(c->W[c->cur], c->plane, c->gas.gam, g, c->lpt.pivots, c->lpt.coef[b],
+                                              c->stream);
     });
 }
 
@@ -896,6 +895,20 @@ struct gkm_ensemble {
     gkm::CflArgs* dcfl = nullptr;     // [parity][member]
     double* partial = nullptr;        // n * kEnsCflBlocks
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // tracers of the members that hold particles ("entries"); the spline of
+    // W[q] lives in coef[q], so both captured steps are fixed
+    int npart = 0;
+    lpt::EnsPart* dpart = nullptr;  // [parity][sample, Heun][entry]
+    int* dpfirst = nullptr;         // [entry + 1] first block of each entry
+    int pblk = 0;
+    lpt::ZArgs* dz = nullptr;       // [parity][entry]: z pass of u, v, w of the next state
+    int* dzfirst = nullptr;         // [entry + 1]
+    int zblk = 0, zchunks = 0;
+    lpt::EnsLine* dline = nullptr;  // [parity][3 entry]: x and y passes of u, v, w
+    int* dlfirst = nullptr;         // [axis - 1][3 entry + 1]
+    int lblk[2] = {0, 0};
+    size_t pcap = 0;                // entries the device tables hold
+    std::vector<char> psig;         // bytes of the uploaded tables
     std::string err;
 };
 
@@ -978,15 +991,134 @@ int ens_build(gkm_ensemble* e, const int t[3])
     return GKM_OK;
 }
 
-// one step of every member from parity p: CFL dt, then the x, y, z sweeps
+// one step of every member from parity p: tracer velocities at x^n from the
+// spline of W^n (update.cpp:136-137), CFL dt, the x, y, z sweeps, the velocity
+// splines of W^{n+1} (update.cpp:157) and the Heun update (ParticleTrack.cpp:48-73)
 hipError_t ens_step(gkm_ensemble* e, int p)
 {
-    const int n = e->n;
-    hipError_t r = gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream);
+    const int n = e->n, np = e->npart;
+    hipError_t r = hipSuccess;
+    if (np) r = lpt::launch_ens_sample(e->dpart + (size_t)(2 * p) * np, e->dpfirst, np, e->pblk, e->stream);
+    if (r == hipSuccess) r = gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream);
     for (int d = 0; d < 3 && r == hipSuccess; d++)
         r = gkm::launch_ens_sweep(d, e->tile[d], e->dargs + ((size_t)p * 3 + d) * n, e->dfirst + (size_t)d * (n + 1),
                                   n, e->nblk[d], e->mixed, e->stream);
+    if (np && r == hipSuccess)
+        r = lpt::launch_ens_prefilter(e->dz + (size_t)p * np, e->dzfirst, np, e->zblk, e->zchunks,
+                                      e->dline + (size_t)p * 3 * np, e->dlfirst, 3 * np, e->lblk, e->stream);
+    if (np && r == hipSuccess)
+        r = lpt::launch_ens_heun(e->dpart + (size_t)(2 * p + 1) * np, e->dpfirst, np, e->pblk, e->stream);
     return r;
+}
+
+// Tracer tables for a run starting at parity p.  Every member with particles
+// gets the spline of its resident state W[p] in coef[p] (built here, outside
+// any capture), then the tables are uploaded if they changed (new particle
+// buffers, members gaining or losing particles), which drops the graphs.
+int ens_particles_prepare(gkm_ensemble* e, int p)
+{
+    std::vector<gkm_ctx*> ent;
+    for (gkm_ctx* c : e->mem)
+        if (c->lpt.np > 0) ent.push_back(c);
+    for (gkm_ctx* c : ent) {
+        lpt::State& L = c->lpt;
+        if (L.valid && L.cur == p) continue;
+        const int rc = build_spline(c, p);
+        if (rc) return efail(e, rc, std::string("member spline: ") + c->err);
+        L.cur = p;
+        L.valid = true;
+    }
+    const int np = (int)ent.size();
+    std::vector<lpt::EnsPart> parts((size_t)4 * np);
+    std::vector<lpt::ZArgs> zs((size_t)2 * np);
+    std::vector<lpt::EnsLine> lines((size_t)6 * np);
+    std::vector<int> pfirst(np + 1), zfirst(np + 1), lfirst((size_t)2 * (3 * np + 1));
+    int acc = 0, zacc = 0, lacc[2] = {0, 0}, ntmax = 0;
+    for (int k = 0; k < np; k++) {
+        gkm_ctx* c = ent[k];
+        lpt::State& L = c->lpt;
+        const lpt::Geom g = geom_of(c);
+        ntmax = std::max(ntmax, g.nt);
+        for (int q = 0; q < 2; q++)
+            for (int h = 0; h < 2; h++) {
+                const int b = h ? 1 - q : q;  // sample on the spline of W[q], Heun on that of W[1-q]
+                parts[(size_t)(2 * q + h) * np + k] =
+                    lpt::EnsPart{L.np, L.pos, L.U, L.G, {L.coef[b][0], L.coef[b][1], L.coef[b][2]}, g, c->dscal};
+            }
+        for (int q = 0; q < 2; q++) {
+            zs[(size_t)q * np + k] = lpt::ZArgs{c->W[1 - q], c->plane, 1, 3, c->gas.gam, g, L.pivots,
+                                                {L.coef[1 - q][0], L.coef[1 - q][1], L.coef[1 - q][2]}};
+            for (int f = 0; f < 3; f++)
+                lines[(size_t)q * 3 * np + 3 * k + f] = lpt::EnsLine{g, L.pivots, L.coef[1 - q][f]};
+        }
+        pfirst[k] = acc;
+        acc += lpt::ens_particle_blocks(L.np);
+        zfirst[k] = zacc;
+        zacc += lpt::ens_z_blocks(g);
+        for (int f = 0; f < 3; f++)
+            for (int ax = 0; ax < 2; ax++) {
+                lfirst[(size_t)ax * (3 * np + 1) + 3 * k + f] = lacc[ax];
+                lacc[ax] += lpt::ens_xy_blocks(g, ax + 1);
+            }
+    }
+    pfirst[np] = acc;
+    zfirst[np] = zacc;
+    for (int ax = 0; ax < 2; ax++) lfirst[(size_t)ax * (3 * np + 1) + 3 * np] = lacc[ax];
+    const int zchunks = np ? lpt::ens_z_chunks(ntmax) : 0;
+    if (np && !zchunks) return efail(e, GKM_EINVAL, "splines need nc <= 1082");
+    std::vector<char> sig;
+    auto put = [&](const void* ptr, size_t bytes) {
+        const char* b = (const char*)ptr;
+        sig.insert(sig.end(), b, b + bytes);
+    };
+    put(&np, sizeof(np));
+    put(parts.data(), sizeof(lpt::EnsPart) * parts.size());
+    put(zs.data(), sizeof(lpt::ZArgs) * zs.size());
+    put(lines.data(), sizeof(lpt::EnsLine) * lines.size());
+    put(pfirst.data(), sizeof(int) * pfirst.size());
+    put(zfirst.data(), sizeof(int) * zfirst.size());
+    put(lfirst.data(), sizeof(int) * lfirst.size());
+    if (sig == e->psig) return GKM_OK;
+    drop_graphs(e);
+    e->psig.clear();
+    e->npart = 0;
+    if ((size_t)np > e->pcap) {
+        for (void* ptr : {(void*)e->dpart, (void*)e->dpfirst, (void*)e->dz, (void*)e->dzfirst, (void*)e->dline,
+                          (void*)e->dlfirst})
+            (void)hipFree(ptr);
+        e->dpart = nullptr;
+        e->dz = nullptr;
+        e->dline = nullptr;
+        e->dpfirst = e->dzfirst = e->dlfirst = nullptr;
+        e->pcap = 0;
+        if (hipMalloc(&e->dpart, sizeof(lpt::EnsPart) * 4 * (size_t)np) != hipSuccess ||
+            hipMalloc(&e->dpfirst, sizeof(int) * (size_t)(np + 1)) != hipSuccess ||
+            hipMalloc(&e->dz, sizeof(lpt::ZArgs) * 2 * (size_t)np) != hipSuccess ||
+            hipMalloc(&e->dzfirst, sizeof(int) * (size_t)(np + 1)) != hipSuccess ||
+            hipMalloc(&e->dline, sizeof(lpt::EnsLine) * 6 * (size_t)np) != hipSuccess ||
+            hipMalloc(&e->dlfirst, sizeof(int) * 2 * (size_t)(3 * np + 1)) != hipSuccess)
+            return efail(e, GKM_ENOMEM, "hipMalloc ensemble tracer tables");
+        e->pcap = (size_t)np;
+    }
+    if (np) {
+        auto up = [&](void* dst, const void* src, size_t bytes) {
+            return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream);
+        };
+        EHIPCHK(e, up(e->dpart, parts.data(), sizeof(lpt::EnsPart) * parts.size()));
+        EHIPCHK(e, up(e->dpfirst, pfirst.data(), sizeof(int) * pfirst.size()));
+        EHIPCHK(e, up(e->dz, zs.data(), sizeof(lpt::ZArgs) * zs.size()));
+        EHIPCHK(e, up(e->dzfirst, zfirst.data(), sizeof(int) * zfirst.size()));
+        EHIPCHK(e, up(e->dline, lines.data(), sizeof(lpt::EnsLine) * lines.size()));
+        EHIPCHK(e, up(e->dlfirst, lfirst.data(), sizeof(int) * lfirst.size()));
+        EHIPCHK(e, hipStreamSynchronize(e->stream));  // the host vectors go out of scope
+    }
+    e->npart = np;
+    e->pblk = acc;
+    e->zblk = zacc;
+    e->zchunks = zchunks;
+    for (int ax = 0; ax < 2; ax++) e->lblk[ax] = lacc[ax];
+    e->psig = std::move(sig);
+    return GKM_OK;
 }
 
 int ens_parity(gkm_ensemble* e, int* p)
@@ -1049,6 +1181,9 @@ void gkm_ensemble_destroy(gkm_ensemble* e)
     (void)hipFree(e->dfirst);
     (void)hipFree(e->dcfl);
     (void)hipFree(e->partial);
+    for (void* ptr : {(void*)e->dpart, (void*)e->dpfirst, (void*)e->dz, (void*)e->dzfirst, (void*)e->dline,
+                      (void*)e->dlfirst})
+        (void)hipFree(ptr);
     if (e->ev[0]) (void)hipEventDestroy(e->ev[0]);
     if (e->ev[1]) (void)hipEventDestroy(e->ev[1]);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1123,6 +1258,9 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
     int p = 0;
     int rc = ens_parity(e, &p);
     if (rc) return rc;
+    rc = ens_particles_prepare(e, p);
+    if (rc) return rc;
+    EHIPCHK(e, hipDeviceSynchronize());  // member streams (splines, uploads) are done before the ensemble stream runs
     if (use_graph) {
         for (int q = 0; q < 2; q++) {
             if (e->graph[q]) continue;
@@ -1154,7 +1292,14 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
     for (int m = 0; m < e->n; m++) {
         gkm_ctx* c = e->mem[m];
         c->cur = p;
-        c->lpt.valid = false;
+        lpt::State& L = c->lpt;
+        if (L.np > 0 && nsteps > 0) {  // the last step left the spline of W[p] in coef[p] and U, G at its start
+            L.cur = p;
+            L.valid = true;
+            L.sampled = true;
+        } else if (nsteps > 0) {
+            L.valid = false;
+        }
         EHIPCHK(e, hipMemcpy(&flags[m], c->dflag, sizeof(int), hipMemcpyDeviceToHost));
     }
     for (int m = 0; m < e->n; m++)

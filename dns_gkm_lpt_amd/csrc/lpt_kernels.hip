@@ -10,6 +10,9 @@
 // tensor product: the line solve is applied along z, then x, then y.
 #include "lpt_kernels.hpp"
 
+#include <initializer_list>
+#include <type_traits>
+
 namespace lpt {
 
 void release(State& s)
@@ -84,85 +87,205 @@ __device__ __forceinline__ double field_at(const double* __restrict__ W, long pl
     return (den * (gam - 1)) * (W[4 * plane + c] / den - 0.5 * Vsqr);
 }
 
-// Pass 1 staged through LDS: one wave solves the z lines of 64 consecutive x
-// knots (one line per lane) in chunks of KC positions.  Each chunk is loaded
-// and stored row by row (64 consecutive z values per row: coalesced), and each
-// lane runs the recurrence of line_solve on its own row in LDS: the same
-// operations in the same order as line_solve.
-template <int KC>
-__global__ __launch_bounds__(64) void prefilter_z_lds_kernel(const double* __restrict__ W, long plane, int q,
-                                                             double gam, Geom g, const double* __restrict__ piv,
-                                                             double* __restrict__ coef)
+// ---------------------------------------------------------------------------
+// z pass: one wave per z line.  Lane l of chunk j holds knot position
+// 64 j + l, so loads and stores are contiguous along z.  Both sweeps of
+// line_solve are first-order linear recurrences,
+//   forward   dp_r = -piv_r dp_{r-1} + piv_r rhs_r      (dp_{-1} = 0)
+//   backward  x_r  = -piv_r x_{r+1}  + dp_r             (x_{n-1} = dp_{n-1}),
+// so each 64-position chunk is an inclusive wave scan of the affine maps
+// x -> A x + B (DPP row shifts and row broadcasts, no LDS) applied to the
+// carry from the previous chunk.  The backward sweep scans lane-reversed
+// chunks.  Results equal line_solve's to rounding (the scan associates the
+// same products differently: measured <= 4 ulp of the line's largest value).
+// ---------------------------------------------------------------------------
+template <int CTRL, int ROW>
+__device__ __forceinline__ double dpp64(double old, double v)
 {
-    __shared__ double tile[64][KC + 1];
-    const int t = threadIdx.x;
-    const int ngb = (g.nt + 63) / 64;
-    const int a = blockIdx.x / ngb, b0 = (blockIdx.x % ngb) * 64;
-    const int nl = g.nt - b0 < 64 ? g.nt - b0 : 64;  // lines in this group
-    const int nc = g.nc, M = g.nt, n = M - 2;
-    const long N = g.N;
-    const bool mine = t < nl;
-    double* row0 = coef + ((long)a * N + b0) * N;   // line l of the group at row0 + l N
-    double c1 = 0.0, cM = 0.0;
-    if (mine) {
-        c1 = field_at(W, plane, q, gam, nc, a, b0 + t, 0);
-        cM = field_at(W, plane, q, gam, nc, a, b0 + t, M - 1);
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROW, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROW, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// compose with the map of the lane CTRL selects (identity where there is none)
+template <int CTRL, int ROW>
+__device__ __forceinline__ void scan_step(double& A, double& B)
+{
+    const double Ap = dpp64<CTRL, ROW>(1.0, A), Bp = dpp64<CTRL, ROW>(0.0, B);
+    B = A * Bp + B;
+    A = A * Ap;
+}
+
+// inclusive scan over the 64 lanes (lane 0's map applied first); all lanes active
+__device__ __forceinline__ void wave_scan(double& A, double& B)
+{
+    scan_step<0x111, 0xf>(A, B);  // row_shr:1
+    scan_step<0x112, 0xf>(A, B);  // row_shr:2
+    scan_step<0x114, 0xf>(A, B);  // row_shr:4
+    scan_step<0x118, 0xf>(A, B);  // row_shr:8
+    scan_step<0x142, 0xa>(A, B);  // row_bcast:15 into rows 1 and 3
+    scan_step<0x143, 0xc>(A, B);  // row_bcast:31 into rows 2 and 3
+}
+
+__device__ __forceinline__ double lane_of(double v, int l)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// line_solve of one z line: f holds the M knot values (zero beyond M), out
+// receives the M + 2 coefficients.  Called by every lane of a full wave.
+template <int K>
+__device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double* __restrict__ piv,
+                                            double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63, n = M - 2;
+    const double c1 = lane_of(f[0], 0);
+    double cM = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (j == (M - 1) / 64) cM = lane_of(f[j], (M - 1) % 64);
+    double carry = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {  // forward: r = p - 1 for position p = 64 j + lane
+        const int r = 64 * j + lane - 1;
+        double A = 1.0, B = 0.0;
+        if (r >= 0 && r < n) {
+            const double pv = piv[r];
+            double rhs = 6.0 * f[j];
+            if (r == 0) rhs -= c1;
+            if (r == n - 1) rhs -= cM;
+            A = -pv;
+            B = pv * rhs;
+        }
+        wave_scan(A, B);
+        f[j] = A * carry + B;
+        carry = lane_of(f[j], 63);
     }
-    // forward sweep: f_{r+1} -> dp_r at position r + 2
-    double dp = 0.0;
-    for (int r0 = 0; r0 < n; r0 += KC) {
-        const int kc = n - r0 < KC ? n - r0 : KC;
-        __syncthreads();
-        if (t < kc)
-            for (int l = 0; l < nl; l++) tile[l][t] = field_at(W, plane, q, gam, nc, a, b0 + l, r0 + 1 + t);
-        __syncthreads();
-        if (mine)
-            for (int j = 0; j < kc; j++) {
-                const int r = r0 + j;
-                double rhs = 6.0 * tile[t][j];
-                if (r == 0) rhs -= c1;
-                if (r == n - 1) rhs -= cM;
-                dp = (rhs - dp) * piv[r];
-                tile[t][j] = dp;
-            }
-        __syncthreads();
-        if (t < kc)
-            for (int l = 0; l < nl; l++) row0[(long)l * N + r0 + 2 + t] = tile[l][t];
+    double xl = 0.0;  // x_{n-1} = dp_{n-1}
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (j == n / 64) xl = lane_of(f[j], n % 64);
+    carry = 0.0;
+    double x0 = 0.0;
+#pragma unroll
+    for (int j = K - 1; j >= 0; j--) {  // backward on reversed chunks: lane l holds r = 64 j + 62 - l
+        const double d = __shfl(f[j], 63 - lane);
+        const int r = 64 * j + 62 - lane;
+        double A = 1.0, B = 0.0;
+        if (r >= 0 && r < n) {
+            A = r == n - 1 ? 0.0 : -piv[r];
+            B = d;
+        }
+        wave_scan(A, B);
+        const double x = A * carry + B;
+        carry = lane_of(x, 63);
+        if (r >= 0 && r < n) out[r + 2] = x;
+        if (j == 0) x0 = lane_of(x, 62);  // x_0 = c_2
     }
-    // backward sweep, chunks from the end: x_r = dp_r - piv[r] x_{r+1}
-    double x = dp;
-    const double xlast = x;
-    for (int r0 = ((n - 1) / KC) * KC; r0 >= 0; r0 -= KC) {
-        const int kc = n - r0 < KC ? n - r0 : KC;
-        __syncthreads();
-        if (t < kc)
-            for (int l = 0; l < nl; l++) tile[l][t] = row0[(long)l * N + r0 + 2 + t];
-        __syncthreads();
-        if (mine)
-            for (int j = kc - 1; j >= 0; j--) {
-                const int r = r0 + j;
-                if (r == n - 1) continue;  // c_{n+1} = dp_{n-1}
-                x = tile[t][j] - piv[r] * x;
-                tile[t][j] = x;
-            }
-        __syncthreads();
-        if (t < kc)
-            for (int l = 0; l < nl; l++) row0[(long)l * N + r0 + 2 + t] = tile[l][t];
-    }
-    if (mine) {
-        double* out = row0 + (long)t * N;
+    if (lane == 0) {
         out[1] = c1;
         out[M] = cM;
-        out[0] = 2.0 * c1 - x;         // x = c_2
-        out[M + 1] = 2.0 * cM - xlast;  // xlast = c_{M-1}
+        out[0] = 2.0 * c1 - x0;
+        out[M + 1] = 2.0 * cM - xl;
+    }
+}
+
+// largest knot count per line a wave holds (17 chunks of 64: nc <= 1082)
+constexpr int kMaxZChunks = 17;
+
+template <int K>
+__device__ __forceinline__ void prefilter_z_line(long line, const ZArgs& z)
+{
+    const int lane = threadIdx.x & 63, nt = z.g.nt, nc = z.g.nc;
+    const int a = (int)(line / nt), b = (int)(line % nt);
+    const long N = z.g.N;
+    const long rowoff = ((long)a * N + b) * N;
+    if (z.nf == 1) {
+        double f[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int p = 64 * j + lane;
+            f[j] = p < nt ? field_at(z.W, z.plane, z.q0, z.gam, nc, a, b, p) : 0.0;
+        }
+        zline_solve<K>(f, nt, z.piv, z.coef[0] + rowoff);
+        return;
+    }
+    double f[3][K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int p = 64 * j + lane;
+        if (p < nt) {
+            const long c = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc + wrapi(p - 3, nc);
+            const double den = z.W[c];
+#pragma unroll
+            for (int q = 0; q < 3; q++) f[q][j] = z.W[(q + 1) * z.plane + c] / den;  // as field_at
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) f[q][j] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) zline_solve<K>(f[q], nt, z.piv, z.coef[q] + rowoff);
+}
+
+// four waves per block, one z line each
+template <int K>
+__global__ __launch_bounds__(256) void prefilter_z_kernel(ZArgs z)
+{
+    const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (line >= (long)z.g.nt * z.g.nt) return;  // uniform over the wave
+    prefilter_z_line<K>(line, z);
+}
+
+// z pass in place over knot values already in the coefficient box (positions
+// [0, nt) of every z line with a, b < nt), as the Laplacian splines need
+template <int K>
+__global__ __launch_bounds__(256) void knots_z_kernel(double* __restrict__ coef, Geom g, const double* __restrict__ piv)
+{
+    const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nt = g.nt, lane = threadIdx.x & 63;
+    if (line >= (long)nt * nt) return;
+    double* row = coef + ((line / nt) * g.N + line % nt) * g.N;
+    double f[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int p = 64 * j + lane;
+        f[j] = p < nt ? row[p] : 0.0;
+    }
+    zline_solve<K>(f, nt, piv, row);  // every lane has read its values before the first store
+}
+
+// smallest instantiated chunk count holding nt knots (0: too many)
+static int z_chunks(int nt)
+{
+    const int need = (nt + 63) / 64;
+    for (int k : {1, 2, 3, 5, 9, kMaxZChunks})
+        if (need <= k) return k;
+    return 0;
+}
+
+template <typename F>
+static hipError_t with_chunks(int k, F&& f)
+{
+    switch (k) {
+    case 1: return f(std::integral_constant<int, 1>());
+    case 2: return f(std::integral_constant<int, 2>());
+    case 3: return f(std::integral_constant<int, 3>());
+    case 5: return f(std::integral_constant<int, 5>());
+    case 9: return f(std::integral_constant<int, 9>());
+    case kMaxZChunks: return f(std::integral_constant<int, kMaxZChunks>());
+    default: return hipErrorInvalidValue;
     }
 }
 
 // in place along z (stride 1; values at the knots of the coefficient box), x
 // (stride N) or y (stride N^2)
-__global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis, const double* __restrict__ piv)
+__device__ __forceinline__ void prefilter_inplace_one(long t, double* __restrict__ coef, const Geom& g, int axis,
+                                                      const double* __restrict__ piv)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long base, st;
     if (axis == 0) {  // z lines: a, b in [0, nt)
         if (t >= (long)g.nt * g.nt) return;
@@ -184,6 +307,11 @@ __global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int 
     line_solve(g.nt, src, line, st, piv);
 }
 
+__global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis, const double* __restrict__ piv)
+{
+    prefilter_inplace_one((long)blockIdx.x * blockDim.x + threadIdx.x, coef, g, axis, piv);
+}
+
 static unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 static hipError_t prefilter_xy(double* coef, const Geom& g, const double* pivots, hipStream_t s)
@@ -197,15 +325,31 @@ static hipError_t prefilter_xy(double* coef, const Geom& g, const double* pivots
     return hipGetLastError();
 }
 
+static hipError_t launch_z(const ZArgs& z, hipStream_t s)
+{
+    const unsigned blocks = blocks_for((long)z.g.nt * z.g.nt, 4);
+    return with_chunks(z_chunks(z.g.nt), [&](auto k) {
+        hipLaunchKernelGGL(prefilter_z_kernel<decltype(k)::value>, dim3(blocks), dim3(256), 0, s, z);
+        return hipGetLastError();
+    });
+}
+
 hipError_t launch_prefilter(const double* W, long plane, int q, double gam, const Geom& g, const double* pivots,
                             double* coef, hipStream_t s)
 {
-    if (q < 0 || q > 4 || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
-    const unsigned groups = (unsigned)(g.nt * ((g.nt + 63) / 64));
-    hipLaunchKernelGGL(prefilter_z_lds_kernel<64>, dim3(groups), dim3(64), 0, s, W, plane, q, gam, g, pivots, coef);
-    hipError_t e = hipGetLastError();
+    if (q < 0 || q > 4 || g.nt != g.nc + 6 || g.N != g.nt + 2 || !z_chunks(g.nt)) return hipErrorInvalidValue;
+    hipError_t e = launch_z(ZArgs{W, plane, q, 1, gam, g, pivots, {coef, nullptr, nullptr}}, s);
     if (e != hipSuccess) return e;
     return prefilter_xy(coef, g, pivots, s);
+}
+
+hipError_t launch_prefilter_velocity(const double* W, long plane, double gam, const Geom& g, const double* pivots,
+                                     double* const coef[3], hipStream_t s)
+{
+    if (g.nt != g.nc + 6 || g.N != g.nt + 2 || !z_chunks(g.nt)) return hipErrorInvalidValue;
+    hipError_t e = launch_z(ZArgs{W, plane, 1, 3, gam, g, pivots, {coef[0], coef[1], coef[2]}}, s);
+    for (int f = 0; f < 3 && e == hipSuccess; f++) e = prefilter_xy(coef[f], g, pivots, s);
+    return e;
 }
 
 // Laplacian of the spline c at every padded knot (eval_UBspline_3d_d_vgl at the
@@ -235,14 +379,16 @@ __global__ void lap_knots_kernel(const double* __restrict__ c, Geom g, double* _
 
 hipError_t launch_lap_spline(const double* c, const Geom& g, const double* pivots, double* lap, hipStream_t s)
 {
-    if (c == lap || g.nt != g.nc + 6 || g.N != g.nt + 2) return hipErrorInvalidValue;
+    if (c == lap || g.nt != g.nc + 6 || g.N != g.nt + 2 || !z_chunks(g.nt)) return hipErrorInvalidValue;
     const long n = (long)g.nt * g.nt * g.nt;
     hipLaunchKernelGGL(lap_knots_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, c, g, lap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(blocks_for((long)g.nt * g.nt, 64)), dim3(64), 0, s, lap, g, 0,
-                       pivots);
-    e = hipGetLastError();
+    e = with_chunks(z_chunks(g.nt), [&](auto k) {
+        hipLaunchKernelGGL(knots_z_kernel<decltype(k)::value>, dim3(blocks_for((long)g.nt * g.nt, 4)), dim3(256), 0, s,
+                           lap, g, pivots);
+        return hipGetLastError();
+    });
     if (e != hipSuccess) return e;
     return prefilter_xy(lap, g, pivots, s);
 }
@@ -285,12 +431,10 @@ __device__ __forceinline__ Loc locate(double p, const Geom& g, bool deriv)
     return L;
 }
 
-__global__ __launch_bounds__(256) void sample_kernel(long np, const double* __restrict__ pos, const double* __restrict__ cu,
-                                                     const double* __restrict__ cv, const double* __restrict__ cw, Geom g,
-                                                     double* __restrict__ U, double* __restrict__ G)
+__device__ __forceinline__ void sample_one(long p, const double* __restrict__ pos, const double* __restrict__ cu,
+                                           const double* __restrict__ cv, const double* __restrict__ cw, const Geom& g,
+                                           double* __restrict__ U, double* __restrict__ G)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
     const Loc Ly = locate(pos[3 * p + 0], g, true);
     const Loc Lx = locate(pos[3 * p + 1], g, true);
     const Loc Lz = locate(pos[3 * p + 2], g, true);
@@ -321,6 +465,14 @@ __global__ __launch_bounds__(256) void sample_kernel(long np, const double* __re
             G[9 * p + 3 * f + 2] = gz;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void sample_kernel(long np, const double* __restrict__ pos, const double* __restrict__ cu,
+                                                     const double* __restrict__ cv, const double* __restrict__ cw, Geom g,
+                                                     double* __restrict__ U, double* __restrict__ G)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) sample_one(p, pos, cu, cv, cw, g, U, G);
 }
 
 hipError_t launch_sample(long np, const double* pos, const double* const cu[3], const Geom& g, double* U, double* G,
@@ -422,13 +574,10 @@ hipError_t launch_props(long np, const double* pos, const int* tag, const double
 
 // Heun RK2 of Particle::updatePosition (ParticleTrack.cpp:48-73): predictor
 // x + U dt (not wrapped) sampled on the new field, corrector, wrap to [0, 2pi].
-__global__ __launch_bounds__(256) void heun_kernel(long np, double* __restrict__ pos, const double* __restrict__ U,
-                                                   const double* __restrict__ cu, const double* __restrict__ cv,
-                                                   const double* __restrict__ cw, Geom g, const double* __restrict__ dtp)
+__device__ __forceinline__ void heun_one(long p, double* __restrict__ pos, const double* __restrict__ U,
+                                         const double* __restrict__ cu, const double* __restrict__ cv,
+                                         const double* __restrict__ cw, const Geom& g, double dt)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const double dt = *dtp;
     const double u = U[3 * p + 0], v = U[3 * p + 1], w = U[3 * p + 2];
     double y = pos[3 * p + 0], x = pos[3 * p + 1], z = pos[3 * p + 2];
     const Loc Ly = locate(y + v * dt, g, false);
@@ -461,11 +610,114 @@ __global__ __launch_bounds__(256) void heun_kernel(long np, double* __restrict__
     }
 }
 
+__global__ __launch_bounds__(256) void heun_kernel(long np, double* __restrict__ pos, const double* __restrict__ U,
+                                                   const double* __restrict__ cu, const double* __restrict__ cv,
+                                                   const double* __restrict__ cw, Geom g, const double* __restrict__ dtp)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) heun_one(p, pos, U, cu, cv, cw, g, *dtp);
+}
+
 hipError_t launch_heun(long np, double* pos, const double* U, const double* const cn[3], const Geom& g,
                        const double* dt, hipStream_t s)
 {
     if (np <= 0) return hipSuccess;
     hipLaunchKernelGGL(heun_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, U, cn[0], cn[1], cn[2], g, dt);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble: the prefilter passes, sampling and Heun of every member in one
+// launch each.  Block b belongs to the entry e with first[e] <= b < first[e+1]
+// (binary search; uniform over the block) and runs the single-context body with
+// that entry's arguments and block index b - first[e].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ens_entry(const int* __restrict__ first, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void ens_prefilter_z_kernel(const ZArgs* __restrict__ zs,
+                                                              const int* __restrict__ first, int n)
+{
+    const int e = ens_entry(first, n, blockIdx.x);
+    const ZArgs z = zs[e];
+    const long line = (long)(blockIdx.x - first[e]) * 4 + (threadIdx.x >> 6);
+    if (line >= (long)z.g.nt * z.g.nt) return;  // uniform over the wave
+    prefilter_z_line<K>(line, z);
+}
+
+__global__ __launch_bounds__(64) void ens_prefilter_inplace_kernel(const EnsLine* __restrict__ lines,
+                                                                   const int* __restrict__ first, int n, int axis)
+{
+    const int e = ens_entry(first, n, blockIdx.x);
+    const EnsLine L = lines[e];
+    prefilter_inplace_one((long)(blockIdx.x - first[e]) * 64 + threadIdx.x, L.coef, L.g, axis, L.piv);
+}
+
+__global__ __launch_bounds__(256) void ens_sample_kernel(const EnsPart* __restrict__ parts, const int* __restrict__ first,
+                                                         int n)
+{
+    const int e = ens_entry(first, n, blockIdx.x);
+    const EnsPart P = parts[e];
+    const long p = (long)(blockIdx.x - first[e]) * 256 + threadIdx.x;
+    if (p < P.np) sample_one(p, P.pos, P.c[0], P.c[1], P.c[2], P.g, P.U, P.G);
+}
+
+__global__ __launch_bounds__(256) void ens_heun_kernel(const EnsPart* __restrict__ parts, const int* __restrict__ first,
+                                                       int n)
+{
+    const int e = ens_entry(first, n, blockIdx.x);
+    const EnsPart P = parts[e];
+    const long p = (long)(blockIdx.x - first[e]) * 256 + threadIdx.x;
+    if (p < P.np) heun_one(p, P.pos, P.U, P.c[0], P.c[1], P.c[2], P.g, *P.dt);
+}
+
+int ens_z_blocks(const Geom& g) { return (int)blocks_for((long)g.nt * g.nt, 4); }
+
+int ens_xy_blocks(const Geom& g, int axis)
+{
+    return (int)blocks_for(axis == 1 ? (long)g.nt * g.N : (long)g.N * g.N, 64);
+}
+
+int ens_z_chunks(int nt_max) { return z_chunks(nt_max); }
+
+int ens_particle_blocks(long np) { return (int)blocks_for(np, 256); }
+
+hipError_t launch_ens_prefilter(const ZArgs* zs, const int* zfirst, int nz, int zblk, int zchunks,
+                                const EnsLine* lines, const int* lfirst, int nl, const int lblk[2], hipStream_t s)
+{
+    if (nz <= 0) return hipSuccess;
+    hipError_t e = with_chunks(zchunks, [&](auto k) {
+        hipLaunchKernelGGL(ens_prefilter_z_kernel<decltype(k)::value>, dim3(zblk), dim3(256), 0, s, zs, zfirst, nz);
+        return hipGetLastError();
+    });
+    for (int axis = 1; axis <= 2 && e == hipSuccess; axis++) {
+        hipLaunchKernelGGL(ens_prefilter_inplace_kernel, dim3(lblk[axis - 1]), dim3(64), 0, s, lines,
+                           lfirst + (axis - 1) * (nl + 1), nl, axis);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t launch_ens_sample(const EnsPart* parts, const int* first, int n, int nblk, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ens_sample_kernel, dim3(nblk), dim3(256), 0, s, parts, first, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_heun(const EnsPart* parts, const int* first, int n, int nblk, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ens_heun_kernel, dim3(nblk), dim3(256), 0, s, parts, first, n);
     return hipGetLastError();
 }
 

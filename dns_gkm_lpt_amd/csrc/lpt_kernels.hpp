@@ -48,6 +48,9 @@ struct Geom {
 // grid (periodic ghosts read by wrap).
 hipError_t launch_prefilter(const double* W, long plane, int q, double gam, const Geom& g, const double* pivots,
                             double* coef, hipStream_t s);
+// the three velocity components at once (rho read once per knot)
+hipError_t launch_prefilter_velocity(const double* W, long plane, double gam, const Geom& g, const double* pivots,
+                                     double* const coef[3], hipStream_t s);
 // spline of the Laplacian of spline c (knot values, then prefilter in place)
 hipError_t launch_lap_spline(const double* c, const Geom& g, const double* pivots, double* lap, hipStream_t s);
 // full particle properties, kPropWidth doubles per particle written to row tag[p];
@@ -65,6 +68,48 @@ hipError_t launch_heun(long np, double* pos, const double* U, const double* cons
 hipError_t sort_by_cell(State& s, const Geom& g, double dx, hipStream_t st);
 // dst[tag[i]] = src[i] for rows of `width` doubles (slot order -> tag order)
 hipError_t scatter_to_tags(long np, const int* tag, const double* src, int width, double* dst, hipStream_t st);
+
+// ---- batched over the members of an ensemble (no reference counterpart) ----
+// One launch covers every entry; entry e owns blocks [first[e], first[e+1]),
+// and its block runs the same body as the single-context kernel.
+// z pass of the state's splines: nf = 1 (field q0 into coef[0]) or nf = 3
+// (q0 = 1: u, v, w into coef[0..2])
+struct ZArgs {
+    const double* W;
+    long plane;
+    int q0, nf;
+    double gam;
+    Geom g;
+    const double* piv;
+    double* coef[3];
+};
+struct EnsLine {  // x and y passes of one spline, in place
+    Geom g;
+    const double* piv;
+    double* coef;
+};
+struct EnsPart {  // one member's tracers
+    long np;
+    double* pos;
+    double* U;
+    double* G;
+    const double* c[3];  // velocity spline to sample (sample) or of the new field (Heun)
+    Geom g;
+    const double* dt;
+};
+// blocks of one entry: z pass (256-thread blocks, a z line per wave), x and y
+// passes (64, axis 1 or 2), particles (256); ens_z_chunks(largest nt) is the
+// z kernel's chunk count (0: nt too large)
+int ens_z_blocks(const Geom& g);
+int ens_xy_blocks(const Geom& g, int axis);
+int ens_z_chunks(int nt_max);
+int ens_particle_blocks(long np);
+// z pass of every ZArgs entry, then the x and y passes of every EnsLine entry;
+// lfirst[(axis - 1) * (nl + 1) + e]
+hipError_t launch_ens_prefilter(const ZArgs* zs, const int* zfirst, int nz, int zblk, int zchunks,
+                                const EnsLine* lines, const int* lfirst, int nl, const int lblk[2], hipStream_t s);
+hipError_t launch_ens_sample(const EnsPart* parts, const int* first, int n, int nblk, hipStream_t s);
+hipError_t launch_ens_heun(const EnsPart* parts, const int* first, int n, int nblk, hipStream_t s);
 
 // flow statistics (PrintStats, update.cpp:688-842); red holds >= kStatBlocks*32 doubles
 constexpr int kStatBlocks = 512;

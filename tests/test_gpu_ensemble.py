@@ -1,13 +1,16 @@
 """Ensemble of independent runs (one launch per direction for all members,
-hipGraph-captured steps).  Pins: every member equals its single run (the
-ensemble kernel runs the same sweep body; measured bit for bit, asserted to
-1e-13 since it is a separate instantiation), graph replays equal eager
-launches bit for bit, autotuning leaves the states unchanged."""
+hipGraph-captured steps, tracers of every member in one launch per phase).
+Pins: every member equals its single run (the ensemble kernels run the same
+bodies; measured bit for bit, asserted to 1e-13 since they are separate
+instantiations), tracers equal the single context's coupled steps, graph
+replays equal eager launches bit for bit, autotuning leaves the states
+unchanged."""
 import numpy as np
 import pytest
 
-from cases import make_case, rel_err
+from cases import make_case, rel_err, torus_dist
 from dns_gkm_lpt_amd import _lib as gl
+from dns_gkm_lpt_amd import ic as icm
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +51,55 @@ def test_members_equal_single_runs(precision, tol):
         print("member", m, "nc", nc, "rel err vs single run", err, "dt", dts[m], hist[-1], flush=True)
         assert np.all(err < tol), (m, err)
         assert abs(dts[m] - hist[-1]) <= 1e-13 * hist[-1]
+
+
+@pytest.mark.parametrize("precision", [gl.GKM_FP64, gl.GKM_MIXED])
+def test_members_with_tracers_equal_single_coupled_runs(precision):
+    """Member 1 holds no tracers; the others advance theirs with the field
+    (sample at x^n, sweeps, spline of W^{n+1}, Heun), eager and replayed steps
+    mixed, and a second upload of tracers between runs."""
+    cs = cases()
+    counts = [300, 0, 1000, 257]
+    pos = [icm.uniform_particles(n, seed=10 + m) for m, n in enumerate(counts)]
+    pos2 = icm.uniform_particles(123, seed=99)
+    with make_ensemble(cs, precision) as e:
+        e.set_tiles((0, 0, 0))
+        for m, n in enumerate(counts):
+            if n:
+                e.members[m].lpt_upload(pos[m])
+        e.run(2, graph=True)
+        e.run(1, graph=False)
+        e.members[3].lpt_upload(pos2)  # new buffers: the tables are rebuilt and the graphs recaptured
+        e.run(2, graph=True)
+        outs = [(e.members[m].download(), e.members[m].lpt_download() if counts[m] else None)
+                for m in range(len(cs))]
+        pst = e.members[0].lpt_stats()
+    for m, (nc, W, P) in enumerate(cs):
+        with gl.Context(nc, P, cfl=0.5, precision=precision) as c:
+            c.set_tiles((0, 0, 0))
+            c.upload(W)
+            if counts[m]:
+                c.lpt_upload(pos[m])
+            for k in range(5):
+                if k == 3 and m == 3:
+                    c.lpt_upload(pos2)
+                dt = c.cfl_dt()
+                if counts[m]:
+                    c.lpt_coupled_step(dt)
+                else:
+                    c.step(dt)
+            ref = c.download()
+            rpos = c.lpt_download() if counts[m] else None
+            rst = c.lpt_stats() if m == 0 else None
+        err = rel_err(outs[m][0], ref)
+        assert np.all(err < 1e-13), (m, err)
+        if counts[m]:
+            dp = np.max(torus_dist(outs[m][1][0], rpos[0]))
+            dv = np.max(np.abs(outs[m][1][1] - rpos[1])) / np.max(np.abs(rpos[1]))
+            print("member", m, "tracers", len(rpos[0]), "max position diff", dp, "velocity", dv, flush=True)
+            assert dp <= 1e-13 and dv <= 1e-13, (m, dp, dv)
+        if m == 0:
+            assert abs(pst.tke - rst.tke) <= 1e-13 * abs(rst.tke)
 
 
 def test_graph_replays_equal_eager_launches():

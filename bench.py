@@ -184,8 +184,9 @@ def run_ensemble(gl, steps, warmup):
     """The batched-ensemble configuration: 64 independent realizations (24 at
     32^3, 24 at 48^3, 16 at 64^3), each with its own seed, k0 in [3, 6], Mt in
     [0.1, 0.6], Re_lambda in [20, 60], mixed precision, 8192 tracers each,
-    tiles autotuned; steps replayed from captured HIP graphs, timed beside the
-    same steps launched eagerly."""
+    tiles autotuned; steps replayed from captured HIP graphs (one launch per
+    phase), timed beside the same steps launched eagerly and as one
+    persistent-kernel launch per step (also graph-replayed)."""
     from dns_gkm_lpt_amd import ic as icm
 
     rng = np.random.default_rng(4)
@@ -205,13 +206,21 @@ def run_ensemble(gl, steps, warmup):
             e.members[m].lpt_upload(icm.uniform_particles(ntr, seed=100 + m))
         tiles, _ = e.autotune(reps=2)
         e.run(warmup, graph=True)
-        for mode, graph in (("graph", True), ("eager", False)):
+        for mode, graph, persistent in (("graph", True, False), ("eager", False, False),
+                                        ("persistent", True, True)):
+            e.set_persistent(persistent)
+            if persistent:
+                e.run(warmup, graph=True)
             t = time.perf_counter()
             e.run(steps, graph=graph)
             el = time.perf_counter() - t
             out[f"ms_per_step_{mode}"] = 1e3 * el / steps
             out[f"member_cell_updates_per_s_{mode}"] = cells * steps / el
             out[f"particle_updates_per_s_{mode}"] = ntr * len(ncs) * steps / el
+        # phase times of one persistent step as workgroup 0 saw the grid barriers
+        names = ["cfl maxima + tracer velocities", "dt", "x sweep", "y sweep", "z sweep",
+                 "spline z pass", "spline x pass", "spline y pass", "Heun"]
+        out["persistent_phase_ms"] = dict(zip(names, (round(float(v), 4) for v in e.persistent_profile())))
         out["sweep_tiles"] = list(tiles)
         out["steps"] = steps
     out["value"] = out["member_cell_updates_per_s_graph"]

@@ -23,6 +23,7 @@ EXPORTS = [
     "gkm_tl_run", "gkm_adjoint_run",
     "gkm_ensemble_create", "gkm_ensemble_destroy", "gkm_ensemble_last_error", "gkm_ensemble_member",
     "gkm_ensemble_set_tiles", "gkm_ensemble_autotune", "gkm_ensemble_run", "gkm_ensemble_dt",
+    "gkm_ensemble_set_persistent", "gkm_ensemble_persistent_profile",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
     "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_properties", "lpt_coupled_step", "lpt_heun",
 ]
@@ -103,6 +104,8 @@ def lib():
         L.gkm_ensemble_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int * 3), C.POINTER(C.c_double * 9)]
         L.gkm_ensemble_run.argtypes = [P, C.c_int, C.c_int]
         L.gkm_ensemble_dt.argtypes = [P, _dp]
+        L.gkm_ensemble_set_persistent.argtypes = [P, C.c_int]
+        L.gkm_ensemble_persistent_profile.argtypes = [P, _dp, C.POINTER(C.c_int)]
         L.gkm_stats_compute.argtypes = [P, C.POINTER(Stats)]
         L.gkm_timing_enable.argtypes = [P, C.c_int]
         L.gkm_timing_read.argtypes = [P, C.POINTER(C.c_double * 8), C.POINTER(C.c_long * 8)]
@@ -364,6 +367,17 @@ class Ensemble:
 
     def run(self, nsteps, graph=True):
         self._chk(lib().gkm_ensemble_run(self._h, nsteps, 1 if graph else 0))
+
+    def set_persistent(self, on=True):
+        """One persistent-kernel launch per step instead of one launch per phase."""
+        self._chk(lib().gkm_ensemble_set_persistent(self._h, 1 if on else 0))
+
+    def persistent_profile(self):
+        """Run one persistent step; ms of each phase (5 or 9 entries)."""
+        ms = np.zeros(9)
+        n = C.c_int(0)
+        self._chk(lib().gkm_ensemble_persistent_profile(self._h, ms, C.byref(n)))
+        return ms[:n.value]
 
     def dt(self):
         out = np.empty(len(self.members))

@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdnsgkm.so")
-SOURCES = ["gkm_kernels.hip", "lpt_kernels.hip", "lpt_sort.hip", "adj_kernels.hip", "gkm_api.hip"]
+SOURCES = ["gkm_kernels.hip", "ens_persist.hip", "lpt_kernels.hip", "lpt_sort.hip", "adj_kernels.hip",
+           "gkm_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -19,23 +20,40 @@ FLAGS = [
 ]
 
 
-def build(resource_usage=False, verbose=False):
-    objs = []
+def build(resource_usage=False, verbose=False, jobs=None):
+    """Compile every source (in parallel, `jobs` at a time) and link the library."""
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if resource_usage:
             cmd.append("-Rpass-analysis=kernel-resource-usage")
+        cmds.append(cmd)
+        objs.append(obj)
+    jobs = jobs or min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1)))
+    running = []
+    for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        objs.append(obj)
+        running.append(subprocess.Popen(cmd))
+        while len(running) >= jobs:
+            _wait_first(running)
+    while running:
+        _wait_first(running)
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
     return LIB
+
+
+def _wait_first(running):
+    proc = running.pop(0)
+    if proc.wait() != 0:
+        for other in running:
+            other.wait()
+        raise subprocess.CalledProcessError(proc.returncode, proc.args)
 
 
 if __name__ == "__main__":

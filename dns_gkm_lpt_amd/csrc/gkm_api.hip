@@ -17,6 +17,7 @@
 
 #include "../../include/dnsgkm.h"
 #include "adj_kernels.hpp"
+#include "ens_persist.hpp"
 #include "gkm_kernels.hpp"
 #include "lpt_kernels.hpp"
 
@@ -909,6 +910,11 @@ struct gkm_ensemble {
     int lblk[2] = {0, 0};
     size_t pcap = 0;                // entries the device tables hold
     std::vector<char> psig;         // bytes of the uploaded tables
+    // persistent mode: the whole step is one launch of ens_step_kernel
+    bool persistent = false;
+    int pgrid = 0;                     // resident workgroups
+    unsigned long long* dbar = nullptr;  // barrier counter, base, timeout flag
+    unsigned long long timeout_ticks = 0;
     std::string err;
 };
 
@@ -994,9 +1000,45 @@ int ens_build(gkm_ensemble* e, const int t[3])
 // one step of every member from parity p: tracer velocities at x^n from the
 // spline of W^n (update.cpp:136-137), CFL dt, the x, y, z sweeps, the velocity
 // splines of W^{n+1} (update.cpp:157) and the Heun update (ParticleTrack.cpp:48-73)
+gkm::PersistStep persist_args(const gkm_ensemble* e, int p)
+{
+    const int n = e->n, np = e->npart;
+    gkm::PersistStep a{};
+    for (int d = 0; d < 3; d++) {
+        a.sweep[d] = e->dargs + ((size_t)p * 3 + d) * n;
+        a.sfirst[d] = e->dfirst + (size_t)d * (n + 1);
+        a.nsweep[d] = e->nblk[d];
+        a.tile[d] = e->tile[d];
+    }
+    a.cfl = e->dcfl + (size_t)p * n;
+    a.partial = e->partial;
+    a.nmem = n;
+    if (np) {
+        a.sample = e->dpart + (size_t)(2 * p) * np;
+        a.heun = e->dpart + (size_t)(2 * p + 1) * np;
+        a.pfirst = e->dpfirst;
+        a.npart = np;
+        a.pblk = e->pblk;
+        a.z = e->dz + (size_t)p * np;
+        a.zfirst = e->dzfirst;
+        a.zblk = e->zblk;
+        a.zchunks = e->zchunks;
+        a.lines = e->dline + (size_t)p * 3 * np;
+        a.lfirst = e->dlfirst;
+        a.lblk[0] = e->lblk[0];
+        a.lblk[1] = e->lblk[1];
+    }
+    a.count = e->dbar;
+    a.base = e->dbar + 1;
+    a.timeout = reinterpret_cast<int*>(e->dbar + 2);
+    a.timeout_ticks = e->timeout_ticks;
+    return a;
+}
+
 hipError_t ens_step(gkm_ensemble* e, int p)
 {
     const int n = e->n, np = e->npart;
+    if (e->persistent) return gkm::launch_persist_step(persist_args(e, p), e->mixed, e->pgrid, e->stream);
     hipError_t r = hipSuccess;
     if (np) r = lpt::launch_ens_sample(e->dpart + (size_t)(2 * p) * np, e->dpfirst, np, e->pblk, e->stream);
     if (r == hipSuccess) r = gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream);
@@ -1182,7 +1224,7 @@ void gkm_ensemble_destroy(gkm_ensemble* e)
     (void)hipFree(e->dcfl);
     (void)hipFree(e->partial);
     for (void* ptr : {(void*)e->dpart, (void*)e->dpfirst, (void*)e->dz, (void*)e->dzfirst, (void*)e->dline,
-                      (void*)e->dlfirst})
+                      (void*)e->dlfirst, (void*)e->dbar})
         (void)hipFree(ptr);
     if (e->ev[0]) (void)hipEventDestroy(e->ev[0]);
     if (e->ev[1]) (void)hipEventDestroy(e->ev[1]);
@@ -1288,6 +1330,15 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
         p = 1 - p;
     }
     EHIPCHK(e, hipStreamSynchronize(e->stream));
+    if (e->persistent) {
+        int timed_out = 0;
+        EHIPCHK(e, hipMemcpy(&timed_out, e->dbar + 2, sizeof(int), hipMemcpyDeviceToHost));
+        if (timed_out) {
+            const int zero = 0;
+            EHIPCHK(e, hipMemcpy(e->dbar + 2, &zero, sizeof(int), hipMemcpyHostToDevice));
+            return efail(e, GKM_EHIP, "persistent step: a grid barrier timed out (workgroups not all resident)");
+        }
+    }
     std::vector<int> flags(e->n, 0);
     for (int m = 0; m < e->n; m++) {
         gkm_ctx* c = e->mem[m];
@@ -1305,6 +1356,65 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
     for (int m = 0; m < e->n; m++)
         if (flags[m] && e->mem[m]->cfg.check_state)
             return efail(e, GKM_ESTATE, "member " + std::to_string(m) + ": non-physical state (rho <= 0, p <= 0 or NaN)");
+    return GKM_OK;
+}
+
+int gkm_ensemble_set_persistent(gkm_ensemble* e, int on)
+{
+    if (!e) return GKM_EINVAL;
+    if (on && !e->dbar) {
+        const int dev = e->mem[0]->cfg.device;
+        e->pgrid = gkm::persist_grid(e->mixed, dev);
+        if (e->pgrid < 1) return efail(e, GKM_EHIP, "persistent step: occupancy query failed");
+        int khz = 0;
+        EHIPCHK(e, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) return efail(e, GKM_EHIP, "persistent step: no wall-clock rate");
+        e->timeout_ticks = 2ull * 1000ull * (unsigned long long)khz;  // 2 s per barrier wait
+        if (hipMalloc(&e->dbar, 3 * sizeof(unsigned long long)) != hipSuccess)
+            return efail(e, GKM_ENOMEM, "hipMalloc barrier");
+        EHIPCHK(e, hipMemset(e->dbar, 0, 3 * sizeof(unsigned long long)));
+    }
+    if (e->persistent != (on != 0)) drop_graphs(e);
+    e->persistent = on != 0;
+    return GKM_OK;
+}
+
+int gkm_ensemble_persistent_profile(gkm_ensemble* e, double ms[9], int* nphases)
+{
+    if (!e || !ms || !nphases) return efail(e, GKM_EINVAL, "null argument");
+    if (!e->persistent) return efail(e, GKM_EINVAL, "the ensemble is not in persistent mode");
+    int p = 0;
+    int rc = ens_parity(e, &p);
+    if (rc) return rc;
+    rc = ens_particles_prepare(e, p);
+    if (rc) return rc;
+    EHIPCHK(e, hipDeviceSynchronize());
+    unsigned long long* dtr = nullptr;
+    if (hipMalloc(&dtr, 10 * sizeof(unsigned long long)) != hipSuccess) return efail(e, GKM_ENOMEM, "hipMalloc trace");
+    gkm::PersistStep a = persist_args(e, p);
+    a.trace = dtr;
+    hipError_t r = gkm::launch_persist_step(a, e->mixed, e->pgrid, e->stream);
+    unsigned long long t[10] = {0};
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    if (r == hipSuccess) r = hipMemcpy(t, dtr, sizeof(t), hipMemcpyDeviceToHost);
+    (void)hipFree(dtr);
+    if (r != hipSuccess) return efail(e, GKM_EHIP, std::string("persistent profile: ") + hipGetErrorString(r));
+    // the traced step advanced every member by one step
+    for (gkm_ctx* c : e->mem) {
+        c->cur = 1 - p;
+        if (c->lpt.np > 0) {
+            c->lpt.cur = 1 - p;
+            c->lpt.valid = true;
+            c->lpt.sampled = true;
+        } else {
+            c->lpt.valid = false;
+        }
+    }
+    int khz = 0;
+    EHIPCHK(e, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->mem[0]->cfg.device));
+    const int nb = e->npart ? 9 : 5;
+    for (int k = 0; k < nb; k++) ms[k] = (double)(t[k + 1] - t[k]) / (double)khz;
+    *nphases = nb;
     return GKM_OK;
 }
 

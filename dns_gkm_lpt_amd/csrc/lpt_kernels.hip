@@ -9,6 +9,7 @@
 // c_0 = 2c_1 - c_2, c_{nt+1} = 2c_nt - c_{nt-1}.  The 3-D coefficients are the
 // tensor product: the line solve is applied along z, then x, then y.
 #include "lpt_kernels.hpp"
+#include "lpt_device.hpp"
 
 #include <initializer_list>
 #include <type_traits>
@@ -37,198 +38,6 @@ void release(State& s)
     (void)hipFree(s.perm);
     (void)hipFree(s.sort_tmp);
     s = State();
-}
-
-__device__ __forceinline__ int wrapi(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
-
-// In-place natural-spline line solve.  The line holds f_0..f_{M-1} at
-// positions 0..M-1 (stride st) and receives c_0..c_{M+1}; Src reads f_p.
-template <typename Src>
-__device__ __forceinline__ void line_solve(int M, const Src& src, double* out, long st, const double* __restrict__ piv)
-{
-    const int n = M - 2;
-    const double c1 = src(0), cM = src(M - 1);
-    double next = src(1);
-    double dp = 0.0;
-    for (int r = 0; r < n; r++) {
-        const double f = next;
-        if (r + 1 < n) next = src(r + 2);  // read before position r+2 is overwritten
-        double rhs = 6.0 * f;
-        if (r == 0) rhs -= c1;
-        if (r == n - 1) rhs -= cM;
-        dp = (rhs - dp) * piv[r];
-        out[(long)(r + 2) * st] = dp;
-    }
-    double x = dp;  // x_{n-1}
-    const double xlast = x;
-    for (int r = n - 2; r >= 0; r--) {
-        x = out[(long)(r + 2) * st] - piv[r] * x;
-        out[(long)(r + 2) * st] = x;
-    }
-    out[st] = c1;
-    out[(long)M * st] = cM;
-    out[0] = 2.0 * c1 - x;                        // x = c_2
-    out[(long)(M + 1) * st] = 2.0 * cM - xlast;   // xlast = c_{M-1}
-}
-
-// Pass 1: z lines straight from the state (periodic wrap).  Source field q as
-// generateSplines forms it (update.cpp:1202-1216): 0 = rho, 1..3 = velocity
-// W_q / rho, 4 = pressure (rho (gam-1)) (E/rho - |U|^2/2) with U = W/rho.
-// Field q of the state at padded position (a, b, k) of the knot grid.
-__device__ __forceinline__ double field_at(const double* __restrict__ W, long plane, int q, double gam, int nc, int a,
-                                          int b, int k)
-{
-    const long c = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc + wrapi(k - 3, nc);
-    const double den = W[c];
-    if (q == 0) return den;
-    if (q <= 3) return W[q * plane + c] / den;
-    const double u = W[plane + c] / den, v = W[2 * plane + c] / den, w = W[3 * plane + c] / den;
-    const double Vsqr = u * u + v * v + w * w;
-    return (den * (gam - 1)) * (W[4 * plane + c] / den - 0.5 * Vsqr);
-}
-
-// ---------------------------------------------------------------------------
-// z pass: one wave per z line.  Lane l of chunk j holds knot position
-// 64 j + l, so loads and stores are contiguous along z.  Both sweeps of
-// line_solve are first-order linear recurrences,
-//   forward   dp_r = -piv_r dp_{r-1} + piv_r rhs_r      (dp_{-1} = 0)
-//   backward  x_r  = -piv_r x_{r+1}  + dp_r             (x_{n-1} = dp_{n-1}),
-// so each 64-position chunk is an inclusive wave scan of the affine maps
-// x -> A x + B (DPP row shifts and row broadcasts, no LDS) applied to the
-// carry from the previous chunk.  The backward sweep scans lane-reversed
-// chunks.  Results equal line_solve's to rounding (the scan associates the
-// same products differently: measured <= 4 ulp of the line's largest value).
-// ---------------------------------------------------------------------------
-template <int CTRL, int ROW>
-__device__ __forceinline__ double dpp64(double old, double v)
-{
-    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROW, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROW, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// compose with the map of the lane CTRL selects (identity where there is none)
-template <int CTRL, int ROW>
-__device__ __forceinline__ void scan_step(double& A, double& B)
-{
-    const double Ap = dpp64<CTRL, ROW>(1.0, A), Bp = dpp64<CTRL, ROW>(0.0, B);
-    B = A * Bp + B;
-    A = A * Ap;
-}
-
-// inclusive scan over the 64 lanes (lane 0's map applied first); all lanes active
-__device__ __forceinline__ void wave_scan(double& A, double& B)
-{
-    scan_step<0x111, 0xf>(A, B);  // row_shr:1
-    scan_step<0x112, 0xf>(A, B);  // row_shr:2
-    scan_step<0x114, 0xf>(A, B);  // row_shr:4
-    scan_step<0x118, 0xf>(A, B);  // row_shr:8
-    scan_step<0x142, 0xa>(A, B);  // row_bcast:15 into rows 1 and 3
-    scan_step<0x143, 0xc>(A, B);  // row_bcast:31 into rows 2 and 3
-}
-
-__device__ __forceinline__ double lane_of(double v, int l)
-{
-    const long long x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// line_solve of one z line: f holds the M knot values (zero beyond M), out
-// receives the M + 2 coefficients.  Called by every lane of a full wave.
-template <int K>
-__device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double* __restrict__ piv,
-                                            double* __restrict__ out)
-{
-    const int lane = threadIdx.x & 63, n = M - 2;
-    const double c1 = lane_of(f[0], 0);
-    double cM = 0.0;
-#pragma unroll
-    for (int j = 0; j < K; j++)
-        if (j == (M - 1) / 64) cM = lane_of(f[j], (M - 1) % 64);
-    double carry = 0.0;
-#pragma unroll
-    for (int j = 0; j < K; j++) {  // forward: r = p - 1 for position p = 64 j + lane
-        const int r = 64 * j + lane - 1;
-        double A = 1.0, B = 0.0;
-        if (r >= 0 && r < n) {
-            const double pv = piv[r];
-            double rhs = 6.0 * f[j];
-            if (r == 0) rhs -= c1;
-            if (r == n - 1) rhs -= cM;
-            A = -pv;
-            B = pv * rhs;
-        }
-        wave_scan(A, B);
-        f[j] = A * carry + B;
-        carry = lane_of(f[j], 63);
-    }
-    double xl = 0.0;  // x_{n-1} = dp_{n-1}
-#pragma unroll
-    for (int j = 0; j < K; j++)
-        if (j == n / 64) xl = lane_of(f[j], n % 64);
-    carry = 0.0;
-    double x0 = 0.0;
-#pragma unroll
-    for (int j = K - 1; j >= 0; j--) {  // backward on reversed chunks: lane l holds r = 64 j + 62 - l
-        const double d = __shfl(f[j], 63 - lane);
-        const int r = 64 * j + 62 - lane;
-        double A = 1.0, B = 0.0;
-        if (r >= 0 && r < n) {
-            A = r == n - 1 ? 0.0 : -piv[r];
-            B = d;
-        }
-        wave_scan(A, B);
-        const double x = A * carry + B;
-        carry = lane_of(x, 63);
-        if (r >= 0 && r < n) out[r + 2] = x;
-        if (j == 0) x0 = lane_of(x, 62);  // x_0 = c_2
-    }
-    if (lane == 0) {
-        out[1] = c1;
-        out[M] = cM;
-        out[0] = 2.0 * c1 - x0;
-        out[M + 1] = 2.0 * cM - xl;
-    }
-}
-
-// largest knot count per line a wave holds (17 chunks of 64: nc <= 1082)
-constexpr int kMaxZChunks = 17;
-
-template <int K>
-__device__ __forceinline__ void prefilter_z_line(long line, const ZArgs& z)
-{
-    const int lane = threadIdx.x & 63, nt = z.g.nt, nc = z.g.nc;
-    const int a = (int)(line / nt), b = (int)(line % nt);
-    const long N = z.g.N;
-    const long rowoff = ((long)a * N + b) * N;
-    if (z.nf == 1) {
-        double f[K];
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int p = 64 * j + lane;
-            f[j] = p < nt ? field_at(z.W, z.plane, z.q0, z.gam, nc, a, b, p) : 0.0;
-        }
-        zline_solve<K>(f, nt, z.piv, z.coef[0] + rowoff);
-        return;
-    }
-    double f[3][K];
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int p = 64 * j + lane;
-        if (p < nt) {
-            const long c = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc + wrapi(p - 3, nc);
-            const double den = z.W[c];
-#pragma unroll
-            for (int q = 0; q < 3; q++) f[q][j] = z.W[(q + 1) * z.plane + c] / den;  // as field_at
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) f[q][j] = 0.0;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 3; q++) zline_solve<K>(f[q], nt, z.piv, z.coef[q] + rowoff);
 }
 
 // four waves per block, one z line each
@@ -279,32 +88,6 @@ static hipError_t with_chunks(int k, F&& f)
     case kMaxZChunks: return f(std::integral_constant<int, kMaxZChunks>());
     default: return hipErrorInvalidValue;
     }
-}
-
-// in place along z (stride 1; values at the knots of the coefficient box), x
-// (stride N) or y (stride N^2)
-__device__ __forceinline__ void prefilter_inplace_one(long t, double* __restrict__ coef, const Geom& g, int axis,
-                                                      const double* __restrict__ piv)
-{
-    long base, st;
-    if (axis == 0) {  // z lines: a, b in [0, nt)
-        if (t >= (long)g.nt * g.nt) return;
-        const int a = (int)(t / g.nt), b = (int)(t % g.nt);
-        base = ((long)a * g.N + b) * g.N;
-        st = 1;
-    } else if (axis == 1) {  // x lines: a in [0, nt), c in [0, N)
-        if (t >= (long)g.nt * g.N) return;
-        const int a = (int)(t / g.N), c = (int)(t % g.N);
-        base = (long)a * g.N * g.N + c;
-        st = g.N;
-    } else {          // y lines: b, c in [0, N)
-        if (t >= (long)g.N * g.N) return;
-        base = t;
-        st = (long)g.N * g.N;
-    }
-    double* line = coef + base;
-    auto src = [&](int p) { return line[(long)p * st]; };
-    line_solve(g.nt, src, line, st, piv);
 }
 
 __global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis, const double* __restrict__ piv)
@@ -391,80 +174,6 @@ hipError_t launch_lap_spline(const double* c, const Geom& g, const double* pivot
     });
     if (e != hipSuccess) return e;
     return prefilter_xy(lap, g, pivots, s);
-}
-
-// ---------------------------------------------------------------------------
-// Evaluation (einspline eval_UBspline_3d_d / _vg semantics): u = (p-start)/delta,
-// i = floor(u), t = u - i, weights of the uniform cubic B-spline.  Indices are
-// clamped to the coefficient box so no input can read outside it; every point
-// of the padded grid range is inside the box unclamped.
-// ---------------------------------------------------------------------------
-struct Loc {
-    int i;
-    double t;
-    double w[4], d[4];
-};
-
-__device__ __forceinline__ Loc locate(double p, const Geom& g, bool deriv)
-{
-    double u = (p - g.start) * g.dinv;
-    double ip = floor(u);
-    double t = u - ip;
-    int i = (int)ip;
-    if (i < 0) { i = 0; t = 0.0; }
-    if (i > g.N - 4) { i = g.N - 4; t = 1.0; }
-    Loc L;
-    L.i = i;
-    L.t = t;
-    const double t2 = t * t, t3 = t2 * t;
-    const double s6 = 1.0 / 6.0;
-    L.w[0] = (-t3 + 3 * t2 - 3 * t + 1) * s6;
-    L.w[1] = (3 * t3 - 6 * t2 + 4) * s6;
-    L.w[2] = (-3 * t3 + 3 * t2 + 3 * t + 1) * s6;
-    L.w[3] = t3 * s6;
-    if (deriv) {
-        L.d[0] = (-0.5 * t2 + t - 0.5) * g.dinv;
-        L.d[1] = (1.5 * t2 - 2 * t) * g.dinv;
-        L.d[2] = (-1.5 * t2 + t + 0.5) * g.dinv;
-        L.d[3] = 0.5 * t2 * g.dinv;
-    }
-    return L;
-}
-
-__device__ __forceinline__ void sample_one(long p, const double* __restrict__ pos, const double* __restrict__ cu,
-                                           const double* __restrict__ cv, const double* __restrict__ cw, const Geom& g,
-                                           double* __restrict__ U, double* __restrict__ G)
-{
-    const Loc Ly = locate(pos[3 * p + 0], g, true);
-    const Loc Lx = locate(pos[3 * p + 1], g, true);
-    const Loc Lz = locate(pos[3 * p + 2], g, true);
-    const long N = g.N;
-    const double* cs[3] = {cu, cv, cw};
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        double v = 0, gy = 0, gx = 0, gz = 0;
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                const double* row = cs[f] + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
-                double rv = 0, rd = 0;
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    rv += Lz.w[c] * row[c];
-                    rd += Lz.d[c] * row[c];
-                }
-                const double wab = Ly.w[a] * Lx.w[b];
-                v += wab * rv;
-                gz += wab * rd;
-                gy += Ly.d[a] * Lx.w[b] * rv;
-                gx += Ly.w[a] * Lx.d[b] * rv;
-            }
-        if (U) U[3 * p + f] = v;
-        if (G) {
-            G[9 * p + 3 * f + 0] = gy;
-            G[9 * p + 3 * f + 1] = gx;
-            G[9 * p + 3 * f + 2] = gz;
-        }
-    }
 }
 
 __global__ __launch_bounds__(256) void sample_kernel(long np, const double* __restrict__ pos, const double* __restrict__ cu,
@@ -572,44 +281,6 @@ hipError_t launch_props(long np, const double* pos, const int* tag, const double
     return hipGetLastError();
 }
 
-// Heun RK2 of Particle::updatePosition (ParticleTrack.cpp:48-73): predictor
-// x + U dt (not wrapped) sampled on the new field, corrector, wrap to [0, 2pi].
-__device__ __forceinline__ void heun_one(long p, double* __restrict__ pos, const double* __restrict__ U,
-                                         const double* __restrict__ cu, const double* __restrict__ cv,
-                                         const double* __restrict__ cw, const Geom& g, double dt)
-{
-    const double u = U[3 * p + 0], v = U[3 * p + 1], w = U[3 * p + 2];
-    double y = pos[3 * p + 0], x = pos[3 * p + 1], z = pos[3 * p + 2];
-    const Loc Ly = locate(y + v * dt, g, false);
-    const Loc Lx = locate(x + u * dt, g, false);
-    const Loc Lz = locate(z + w * dt, g, false);
-    const long N = g.N;
-    const double* cs[3] = {cu, cv, cw};
-    double U2[3];
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        double val = 0;
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                const double* row = cs[f] + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
-                const double rv = Lz.w[0] * row[0] + Lz.w[1] * row[1] + Lz.w[2] * row[2] + Lz.w[3] * row[3];
-                val += Ly.w[a] * Lx.w[b] * rv;
-            }
-        U2[f] = val;
-    }
-    y += 0.5 * (v + U2[1]) * dt;
-    x += 0.5 * (u + U2[0]) * dt;
-    z += 0.5 * (w + U2[2]) * dt;
-    const double twopi = 2.0 * 3.141592653589793238462643383279502884;
-    double r[3] = {y, x, z};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        if (r[c] > twopi) r[c] = fmod(r[c], twopi);
-        if (r[c] < 0) r[c] = twopi - fmod(-r[c], twopi);
-        pos[3 * p + c] = r[c];
-    }
-}
-
 __global__ __launch_bounds__(256) void heun_kernel(long np, double* __restrict__ pos, const double* __restrict__ U,
                                                    const double* __restrict__ cu, const double* __restrict__ cv,
                                                    const double* __restrict__ cw, Geom g, const double* __restrict__ dtp)
@@ -632,17 +303,6 @@ hipError_t launch_heun(long np, double* pos, const double* U, const double* cons
 // (binary search; uniform over the block) and runs the single-context body with
 // that entry's arguments and block index b - first[e].
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int ens_entry(const int* __restrict__ first, int n, int b)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 template <int K>
 __global__ __launch_bounds__(256) void ens_prefilter_z_kernel(const ZArgs* __restrict__ zs,
                                                               const int* __restrict__ first, int n)

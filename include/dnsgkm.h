@@ -153,6 +153,8 @@ int gkm_adjoint_run(gkm_ctx *ctx, double dt, int nsteps, int every, const double
  * CFL dt), stepped together: per step one launch computes every member's dt
  * and one launch per direction sweeps every member; with use_graph the step
  * is a captured HIP graph (one per buffer parity) replayed nsteps times.
+ * Tracers uploaded to a member (lpt_upload on gkm_ensemble_member) advance
+ * with it exactly as lpt_coupled_step advances them in a single context.
  * Members must share precision and device.  A member's state goes in and out
  * through gkm_upload_state / gkm_download_state / gkm_stats_compute on
  * gkm_ensemble_member(e, m) (owned by the ensemble; do not destroy it, and do
@@ -170,6 +172,16 @@ int gkm_ensemble_set_tiles(gkm_ensemble *e, const int tile[3]);
 int gkm_ensemble_autotune(gkm_ensemble *e, int reps, int tile[3], double ms[9]);
 /* nsteps steps of every member, each with dt from its own CFL condition */
 int gkm_ensemble_run(gkm_ensemble *e, int nsteps, int use_graph);
+/* on = 1: each step is one launch of a persistent kernel (as many workgroups
+ * as stay resident; every phase -- CFL, x, y, z sweeps, tracer splines and
+ * Heun -- takes its work items from the same tables, phases separated by grid
+ * barriers with a bounded wait); results equal the per-phase launches' bit
+ * for bit.  A barrier that timed out makes gkm_ensemble_run fail. */
+int gkm_ensemble_set_persistent(gkm_ensemble *e, int on);
+/* one persistent step (eager) with the wall-clock time of each phase, as
+ * workgroup 0 saw the barriers: *nphases = 5 (CFL maxima + tracer velocities,
+ * dt, x, y, z sweeps) or 9 (+ spline z, x, y passes, Heun) entries of ms */
+int gkm_ensemble_persistent_profile(gkm_ensemble *e, double ms[9], int *nphases);
 /* the dt of each member's last step (n doubles) */
 int gkm_ensemble_dt(gkm_ensemble *e, double *dt);
 

@@ -102,6 +102,38 @@ def test_members_with_tracers_equal_single_coupled_runs(precision):
             assert abs(pst.tke - rst.tke) <= 1e-13 * abs(rst.tke)
 
 
+@pytest.mark.parametrize("precision", [gl.GKM_FP64, gl.GKM_MIXED])
+def test_persistent_step_equals_per_phase_launches(precision):
+    """One persistent-kernel launch per step (grid barriers between the CFL,
+    sweep and tracer phases) gives the per-phase launches' results bit for bit:
+    fields, tracer positions and velocities, dt; replayed graphs and eager
+    launches, over several runs (the barrier counter carries across launches)."""
+    cs = cases()
+    counts = [300, 0, 1000, 257]
+    pos = [icm.uniform_particles(n, seed=30 + m) for m, n in enumerate(counts)]
+
+    def run(persistent):
+        with make_ensemble(cs, precision) as e:
+            e.set_tiles((2, 0, 1))
+            for m, n in enumerate(counts):
+                if n:
+                    e.members[m].lpt_upload(pos[m])
+            e.set_persistent(persistent)
+            e.run(3, graph=True)
+            e.run(2, graph=False)
+            e.run(2, graph=True)
+            return ([mb.download() for mb in e.members],
+                    [e.members[m].lpt_download() if counts[m] else None for m in range(len(cs))], e.dt())
+
+    ref, got = run(False), run(True)
+    assert np.array_equal(ref[2], got[2])
+    for m in range(len(cs)):
+        assert np.array_equal(ref[0][m], got[0][m]), m
+        if counts[m]:
+            assert np.array_equal(ref[1][m][0], got[1][m][0]), m
+            assert np.array_equal(ref[1][m][1], got[1][m][1]), m
+
+
 def test_graph_replays_equal_eager_launches():
     cs = cases()
     with make_ensemble(cs, gl.GKM_MIXED) as eager, make_ensemble(cs, gl.GKM_MIXED) as graph:

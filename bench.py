@@ -55,6 +55,16 @@ MEASURED_TRAFFIC_GIB = {
     ("fp64", 1, 2, 512): (70.76, 27.62),
     ("fp64", 2, 0, 512): (75.65, 29.16),
 }
+# rocprofv3 VALUBusy (% of time the VALU issues) of each sweep launch, keyed
+# like MEASURED_TRAFFIC_GIB; commands and table in profiles/sweep_valu_512.md.
+MEASURED_VALU_BUSY = {
+    ("mixed", 0, 2, 512): 83.9,
+    ("mixed", 1, 2, 512): 75.7,
+    ("mixed", 2, 0, 512): 87.2,
+    ("fp64", 0, 0, 512): 58.1,
+    ("fp64", 1, 2, 512): 52.3,
+    ("fp64", 2, 0, 512): 54.9,
+}
 KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
                 3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",
                 6: "particle cell sort"}
@@ -311,7 +321,11 @@ def main():
                      "algorithmic_bytes_per_cell": SWEEP_BYTES[dom],
                      "valu": {"alg_ops_per_face": ALG_OPS_PER_FACE, "achieved_tops": alg_tops,
                               "fp64_peak_tflops": FP64_PEAK_TFLOPS, "frac_of_fp64_peak": alg_tops / FP64_PEAK_TFLOPS,
-                              "note": "reference operation count (one WENO per interface) over the kernel time"}},
+                              "note": "reference operation count (one WENO per interface) over the kernel time",
+                              "valu_busy_pct_rocprof": MEASURED_VALU_BUSY.get(
+                                  (precs[-1], dom, head["sweep_tiles"][dom], nc)),
+                              "valu_busy_note": "rocprofv3 VALUBusy of this kernel configuration "
+                                                "(profiles/sweep_valu_512.md)"}},
         "step_hbm_fraction": STEP_BYTES * head["value"] / 1e9 / HBM_PEAK_GBS,
         "kernels": head["kernels"],
     }

@@ -11,9 +11,10 @@ The recipe follows InitialConditions/init_DNS.m of the reference:
     second-order periodic central differences (init_DNS.m:96-120),
   * eddy time lambda / u_rms (init_DNS.m:124-126),
   * uniform density rho0 = 1 and pressure p0 = rho0 R T0, T0 = 300 K, gamma = 7/5.
-Two deliberate departures, both named in BASELINE's recipe: the random field is
-seeded Gaussian (numpy default_rng) instead of MATLAB's unseeded uniform rand,
-and the spectrum uses |u_k|^2 where init_DNS.m:54 squares the complex values.
+Two deliberate departures: the random field is seeded Gaussian (numpy
+default_rng) instead of MATLAB's unseeded uniform rand, so runs are
+reproducible, and the spectrum uses |u_k|^2 where init_DNS.m:54 squares the
+complex values.
 
 Arrays are indexed [y, x, z] (j, i, k), z fastest, as the solver stores them.
 """

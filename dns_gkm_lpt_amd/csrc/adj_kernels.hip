@@ -10,8 +10,9 @@
 //   tangent-linear: one dual pass per face with the perturbation as tangent;
 //   adjoint: per face, 50 dual passes (one per flux input) contracted with the
 //   face's adjoint flux; the slopes are linear in the WENO states, so their
-//   adjoint is an explicit gather; per face and line, 30 dual passes of WENO
-//   contracted with the adjoint WENO states; then a gather per cell.
+//   adjoint is an explicit gather; per face and line, WENO in reverse mode
+//   (one pass per one-dimensional reconstruction, the energy slot by the
+//   chain rule); then a gather per cell.
 // One thread per face or cell; these kernels favour clarity over speed (they
 // serve the 64^3 adjoint configuration).
 //
@@ -20,6 +21,7 @@
 // kernels (x: n = x, a = y, b = z; y: n = y, a = x, b = z; z: n = z, a = x, b = y).
 #include "adj_kernels.hpp"
 
+#include "adj_math.hpp"
 #include "gkm_dual.hpp"
 #include "gkm_flux.hpp"
 
@@ -248,7 +250,14 @@ __global__ __launch_bounds__(256, 2) void adj_assemble_kernel(const double* __re
     }
 }
 
-// Sbar[5 t + q][f] = sum_o WSbar_o dWS_o / dS_{t,q} (30 dual passes of WENO)
+// Sbar[5 t + q][f] = sum_o WSbar_o dWS_o / dS_{t,q}.  WENO-SYMOO reconstructs
+// rho, rho u, rho v, rho w one component at a time; only the energy slot
+// couples them, E = W0 R T igm + |W1..3|^2 / (2 W0) with T the reconstruction of
+// the cells' temperatures.  The adjoints of the energy slots are first moved
+// onto the other four slots and the reconstructed temperatures (closed-form
+// partial derivatives), then each one-dimensional reconstruction is
+// differentiated in reverse mode (weno_side_grad: all six cells in one pass),
+// and the temperature gradient reaches the cells through T(W).
 template <int DIR>
 __global__ __launch_bounds__(256, 2) void weno_adj_kernel(const double* __restrict__ W, const double* __restrict__ WSbar,
                                                        long plane, int nc, Gas<double> g, double* __restrict__ Sbar)
@@ -257,22 +266,62 @@ __global__ __launch_bounds__(256, 2) void weno_adj_kernel(const double* __restri
     if (f >= plane) return;
     int n, a, b;
     coords_of<DIR>(f, nc, n, a, b);
-    double S[6][5], wb[10];
+    double S[6][5], Tc[6];
+#pragma unroll
     for (int t = 0; t < 6; t++) {
         const long c = cell_at<DIR>(wrap(n - 2 + t, nc), a, b, nc);
+#pragma unroll
         for (int q = 0; q < 5; q++) S[t][q] = W[q * plane + c];
+        Tc[t] = temp_of(S[t], g.gam, g.R);
     }
-    for (int o = 0; o < 10; o++) wb[o] = WSbar[o * plane + f];
-    const Gas<Dual> gd = dual_gas(g);
-#pragma unroll 1
-    for (int k = 0; k < 30; k++) {
-        Dual Sd[6][5], Wl[5], Wr[5];
-        for (int t = 0; t < 6; t++)
-            for (int q = 0; q < 5; q++) Sd[t][q] = Dual(S[t][q], 5 * t + q == k ? 1.0 : 0.0);
-        weno5<Dual>(Sd, gd, Wl, Wr);
-        double s = 0.0;
-        for (int q = 0; q < 5; q++) s += wb[q] * Wl[q].d + wb[5 + q] * Wr[q].d;
-        Sbar[(long)k * plane + f] = s;
+    const double eps = g.eps, igm = 1.0 / (g.gam - 1.0), gR = (g.gam - 1.0) / g.R;
+    double gS[6][5], gT[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        gT[t] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) gS[t][q] = 0.0;
+    }
+    // side 0: the left state Wl (stencil i-2..i+3); side 1: the right state Wr
+    // (the same formula on the mirrored stencil)
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        auto cell = [&](int k) { return side == 0 ? k : 5 - k; };
+        double w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            w[q] = weno_side(S[cell(0)][q], S[cell(1)][q], S[cell(2)][q], S[cell(3)][q], S[cell(4)][q], S[cell(5)][q],
+                             eps);
+        const double T = weno_side(Tc[cell(0)], Tc[cell(1)], Tc[cell(2)], Tc[cell(3)], Tc[cell(4)], Tc[cell(5)], eps);
+        const double eb = WSbar[(5 * side + 4) * plane + f];
+        const double m2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double wbar = WSbar[(5 * side + q) * plane + f] +
+                                eb * (q == 0 ? g.R * T * igm - 0.5 * m2 / (w[0] * w[0]) : w[q] / w[0]);
+            double gq[6] = {0, 0, 0, 0, 0, 0};
+            weno_side_grad(S[cell(0)][q], S[cell(1)][q], S[cell(2)][q], S[cell(3)][q], S[cell(4)][q], S[cell(5)][q],
+                           eps, wbar, gq);
+#pragma unroll
+            for (int k = 0; k < 6; k++) gS[cell(k)][q] += gq[k];
+        }
+        double gt[6] = {0, 0, 0, 0, 0, 0};
+        weno_side_grad(Tc[cell(0)], Tc[cell(1)], Tc[cell(2)], Tc[cell(3)], Tc[cell(4)], Tc[cell(5)], eps,
+                       eb * (w[0] * g.R * igm), gt);
+#pragma unroll
+        for (int k = 0; k < 6; k++) gT[cell(k)] += gt[k];
+    }
+    // T(W) = (gam-1)/R (W4/W0 - |W1..3|^2/(2 W0^2)) per cell
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const double ir = 1.0 / S[t][0], k2 = S[t][1] * S[t][1] + S[t][2] * S[t][2] + S[t][3] * S[t][3];
+        const double dT = gT[t] * gR;
+        gS[t][0] += dT * (-S[t][4] * ir * ir + k2 * ir * ir * ir);
+#pragma unroll
+        for (int q = 1; q < 4; q++) gS[t][q] += dT * (-S[t][q] * ir * ir);
+        gS[t][4] += dT * ir;
+#pragma unroll
+        for (int q = 0; q < 5; q++) Sbar[(long)(5 * t + q) * plane + f] = gS[t][q];
     }
 }
 

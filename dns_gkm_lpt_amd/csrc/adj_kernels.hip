@@ -8,11 +8,12 @@
 // F_d(i-1/2)).  Derivatives come from running the same templates on dual
 // numbers (gkm_dual.hpp):
 //   tangent-linear: one dual pass per face with the perturbation as tangent;
-//   adjoint: per face, 50 dual passes (one per flux input) contracted with the
-//   face's adjoint flux; the slopes are linear in the WENO states, so their
-//   adjoint is an explicit gather; per face and line, WENO in reverse mode
-//   (one pass per one-dimensional reconstruction, the energy slot by the
-//   chain rule); then a gather per cell.
+//   adjoint: per face, the flux's 50 partials contracted with the face's
+//   adjoint flux (face_flux_adjoint: one reverse pass for the inputs in which
+//   the flux is affine, 11 dual passes for the rest); the slopes are linear in
+//   the WENO states, so their adjoint is an explicit gather; per face and line, WENO in
+//   reverse mode (one pass per one-dimensional reconstruction, the energy slot
+//   by the chain rule); then a gather per cell.
 // One thread per face or cell; these kernels favour clarity over speed (they
 // serve the 64^3 adjoint configuration).
 //
@@ -55,26 +56,6 @@ __device__ __forceinline__ int phys(int q)
 {
     constexpr int sw = DIR == 1 ? 2 : 3;
     return DIR == 0 ? q : (q == 1 ? sw : (q == sw ? 1 : q));
-}
-
-// flux of one face in the rotated frame from the 50 inputs x = (WL, WR, Wl, Wr,
-// D[6]) (each 5 values, rotated frame), collision time from WL
-template <typename T>
-__device__ __forceinline__ void face_flux(const T x[50], T dt, T dx, const Gas<T>& g, T F[5])
-{
-    const T* WL = x;
-    const T tau = collision_time<T>(WL, g);
-    T D[6][5];
-#pragma unroll
-    for (int d = 0; d < 6; d++)
-#pragma unroll
-        for (int q = 0; q < 5; q++) D[d][q] = x[20 + 5 * d + q];
-    gks_flux_tf<T, T>(x, x + 5, x + 10, x + 15, D, tau, dt, dx, g, F);
-}
-
-__host__ __device__ inline Gas<Dual> dual_gas(const Gas<double>& g)
-{
-    return Gas<Dual>{Dual(g.gam), Dual(g.R), Dual(g.inv_prn), Dual(g.mu0), Dual(g.T0), Dual(g.eps), g.K};
 }
 
 // the 50 flux inputs of face (n, a, b) in the rotated frame: values (and tangents
@@ -210,17 +191,9 @@ __global__ __launch_bounds__(256, 2) void flux_adj_kernel(const double* __restri
     auto get = [&](int arr, int v, long idx) { return arr == 0 ? W[v * plane + idx] : WS[v * plane + idx]; };
     double x[50];
     gather_inputs<DIR, double>(n, a, b, nc, plane, dx, get, x);
-    const Gas<Dual> gd = dual_gas(g);
-#pragma unroll 1
-    for (int k = 0; k < 50; k++) {
-        Dual xd[50], F[5];
-        for (int i = 0; i < 50; i++) xd[i] = Dual(x[i], i == k ? 1.0 : 0.0);
-        face_flux<Dual>(xd, Dual(dt), Dual(dx), gd, F);
-        double s = 0.0;
-        for (int q = 0; q < 5; q++) s += Fbar[q] * F[q].d;
-        const int grp = k / 5, p = phys<DIR>(k % 5);
-        Xbar[(long)(5 * grp + p) * plane + f] = s;
-    }
+    double xbar[50];
+    face_flux_adjoint(x, dt, dx, g, Fbar, xbar);
+    for (int k = 0; k < 50; k++) Xbar[(long)(5 * (k / 5) + phys<DIR>(k % 5)) * plane + f] = xbar[k];
 }
 
 // adjoints of the WENO states (WSbar, 10 per face) and of the face's two cells

@@ -73,6 +73,75 @@ __host__ __device__ __forceinline__ T weno_side(T a, T b, T c, T d, T e, T f, T 
     return (n0 * q0 + n1 * q1 + n2 * q2 + n3 * q3) / (n0 + n1 + n2 + n3);
 }
 
+// Pieces of weno_pair.  wmax: the larger of two indicators (fmax for float
+// and double: one instruction, equal to a > b ? a : b for non-NaN values).
+// wratio: the weights' normalising division; FP32 (mixed precision, device)
+// multiplies by the hardware reciprocal (1 ulp) instead of dividing.
+template <typename T>
+__host__ __device__ __forceinline__ T wmax(T a, T b) { return a > b ? a : b; }
+template <> __host__ __device__ __forceinline__ float wmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> __host__ __device__ __forceinline__ double wmax<double>(double a, double b) { return fmax(a, b); }
+template <typename T>
+__host__ __device__ __forceinline__ T wratio(T n, T d) { return n / d; }
+__device__ __forceinline__ float wratio(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
+
+// nonlinear combination of the four candidates (weno_side's second half)
+template <typename T>
+__host__ __device__ __forceinline__ T weno_combine(T q0, T q1, T q2, T q3, T s0, T s1, T s2, T s3, T eps)
+{
+    s3 = wmax(wmax(wmax(s0, s1), s2), s3);  // update.cpp:577-584
+    const T D0 = eps + s0, D1 = eps + s1, D2 = eps + s2, D3 = eps + s3;
+    const T p01 = D0 * D1, p23 = D2 * D3;
+    const T n0 = T(0.05) * D1 * p23, n1 = T(0.45) * D0 * p23;
+    const T n2 = T(0.45) * p01 * D3, n3 = T(0.05) * p01 * D2;
+    return wratio(n0 * q0 + n1 * q1 + n2 * q2 + n3 * q3, n0 + n1 + n2 + n3);
+}
+
+// Both reconstructions of one face in one pass: wl = weno_side(a..f) and
+// wr = weno_side(f..a), the same formulas with the rational coefficients
+// folded into single constants (b * 7/6 rather than 7 b / 6, and half of each
+// first-difference combination squared in place of a quarter of its square)
+// and the four 13/12-weighted squared second differences, which the two sides
+// share, formed once.  Equal to the two weno_side calls up to rounding.
+template <typename T>
+__host__ __device__ __forceinline__ void weno_pair(T a, T b, T c, T d, T e, T f, T eps, T& wl, T& wr)
+{
+    const T k3 = T(1.0 / 3.0), k6 = T(1.0 / 6.0), k56 = T(5.0 / 6.0), k76 = T(7.0 / 6.0), k116 = T(11.0 / 6.0);
+    const T k13 = T(13.0 / 12.0);
+    const T t0 = a - T(2) * b + c, t1 = b - T(2) * c + d, t2 = c - T(2) * d + e, t3 = d - T(2) * e + f;
+    const T K0 = k13 * t0 * t0, K1 = k13 * t1 * t1, K2 = k13 * t2 * t2, K3 = k13 * t3 * t3;
+    {
+        const T q0 = a * k3 - b * k76 + c * k116;
+        const T q1 = c * k56 - b * k6 + d * k3;
+        const T q2 = c * k3 + d * k56 - e * k6;
+        const T q3 = d * k116 - e * k76 + f * k3;
+        const T h0 = T(0.5) * a - T(2) * b + T(1.5) * c;
+        const T h1 = T(0.5) * (d - b);
+        const T h2 = T(2) * d - T(1.5) * c - T(0.5) * e;
+        const T h3 = T(4) * e - T(2.5) * d - T(1.5) * f;
+        wl = weno_combine(q0, q1, q2, q3, K0 + h0 * h0, K1 + h1 * h1, K2 + h2 * h2, K3 + h3 * h3, eps);
+    }
+    {
+        const T q0 = f * k3 - e * k76 + d * k116;
+        const T q1 = d * k56 - e * k6 + c * k3;
+        const T q2 = d * k3 + c * k56 - b * k6;
+        const T q3 = c * k116 - b * k76 + a * k3;
+        const T h0 = T(0.5) * f - T(2) * e + T(1.5) * d;
+        const T h1 = T(0.5) * (c - e);
+        const T h2 = T(2) * c - T(1.5) * d - T(0.5) * b;
+        const T h3 = T(4) * b - T(2.5) * c - T(1.5) * a;
+        wr = weno_combine(q0, q1, q2, q3, K3 + h0 * h0, K2 + h1 * h1, K1 + h2 * h2, K0 + h3 * h3, eps);
+    }
+}
+
+// temperature with (gam - 1)/R given (one division)
+template <typename T>
+__host__ __device__ __forceinline__ T temp_gr(const T W[5], T gR)
+{
+    const T ir = T(1) / W[0];
+    return gR * (W[4] - T(0.5) * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) * ir) * ir;
+}
+
 // temperature of a conservative state (update.cpp:624-629)
 template <typename T>
 __host__ __device__ __forceinline__ T temp_of(const T W[5], T gam, T R)
@@ -82,22 +151,22 @@ __host__ __device__ __forceinline__ T temp_of(const T W[5], T gam, T R)
 }
 
 // ApplyWENO (update.cpp:556-680): S[s][q] for cells i-2..i+3, out Wl, Wr.
+// Temperatures with (gam-1)/R formed once, and the energy rebuilt with
+// R/(gam-1); both equal the reference's expressions up to rounding.
 template <typename T>
 __host__ __device__ __forceinline__ void weno5(const T S[6][5], const Gas<T>& g, T Wl[5], T Wr[5])
 {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        Wl[q] = weno_side(S[0][q], S[1][q], S[2][q], S[3][q], S[4][q], S[5][q], g.eps);
-        Wr[q] = weno_side(S[5][q], S[4][q], S[3][q], S[2][q], S[1][q], S[0][q], g.eps);
-    }
+    for (int q = 0; q < 4; q++) weno_pair(S[0][q], S[1][q], S[2][q], S[3][q], S[4][q], S[5][q], g.eps, Wl[q], Wr[q]);
+    const T gR = (g.gam - T(1)) / g.R;
     T Tc[6];
 #pragma unroll
-    for (int s = 0; s < 6; s++) Tc[s] = temp_of(S[s], g.gam, g.R);
-    T Tl = weno_side(Tc[0], Tc[1], Tc[2], Tc[3], Tc[4], Tc[5], g.eps);
-    T Tr = weno_side(Tc[5], Tc[4], Tc[3], Tc[2], Tc[1], Tc[0], g.eps);
-    T igm = T(1) / (g.gam - T(1));
-    Wl[4] = Wl[0] * g.R * Tl * igm + T(0.5) * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
-    Wr[4] = Wr[0] * g.R * Tr * igm + T(0.5) * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+    for (int s = 0; s < 6; s++) Tc[s] = temp_gr(S[s], gR);
+    T Tl, Tr;
+    weno_pair(Tc[0], Tc[1], Tc[2], Tc[3], Tc[4], Tc[5], g.eps, Tl, Tr);
+    const T Rg = g.R / (g.gam - T(1));
+    Wl[4] = Wl[0] * Rg * Tl + T(0.5) * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
+    Wr[4] = Wr[0] * Rg * Tr + T(0.5) * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
 }
 
 // collision time mu(T)/p of the face's left cell, Sutherland law (update.cpp:276-278)
@@ -415,33 +484,33 @@ __host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], 
 // and for every slope-dependent moment term of the flux.
 // ===========================================================================
 
-// WENO on FP64 stencils, FP32 arithmetic on differences from the base cell
-// (the reconstruction is shift invariant: candidates are affine with unit
-// coefficient sum and the indicators see differences only).
+// WENO on FP64 stencils, FP32 arithmetic on differences from cell i, the base
+// of both sides (the reconstruction is shift invariant: candidates are affine
+// with unit coefficient sum and the indicators see differences only).
 __host__ __device__ __forceinline__ void weno5_mixed(const double S[6][5], const Gas<double>& g, double Wl[5],
                                                      double Wr[5])
 {
     const float eps = (float)g.eps;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const double c = S[2][q], d = S[3][q];
-        Wl[q] = c + (double)weno_side<float>((float)(S[0][q] - c), (float)(S[1][q] - c), 0.f, (float)(d - c),
-                                             (float)(S[4][q] - c), (float)(S[5][q] - c), eps);
-        Wr[q] = d + (double)weno_side<float>((float)(S[5][q] - d), (float)(S[4][q] - d), 0.f, (float)(c - d),
-                                             (float)(S[1][q] - d), (float)(S[0][q] - d), eps);
+        const double c = S[2][q];
+        float l, r;
+        weno_pair<float>((float)(S[0][q] - c), (float)(S[1][q] - c), 0.f, (float)(S[3][q] - c), (float)(S[4][q] - c),
+                         (float)(S[5][q] - c), eps, l, r);
+        Wl[q] = c + (double)l;
+        Wr[q] = c + (double)r;
     }
+    const double gR = (g.gam - 1.0) / g.R;
     double Tc[6];
 #pragma unroll
-    for (int s = 0; s < 6; s++) Tc[s] = temp_of<double>(S[s], g.gam, g.R);
-    const double Tl = Tc[2] + (double)weno_side<float>((float)(Tc[0] - Tc[2]), (float)(Tc[1] - Tc[2]), 0.f,
-                                                       (float)(Tc[3] - Tc[2]), (float)(Tc[4] - Tc[2]),
-                                                       (float)(Tc[5] - Tc[2]), eps);
-    const double Tr = Tc[3] + (double)weno_side<float>((float)(Tc[5] - Tc[3]), (float)(Tc[4] - Tc[3]), 0.f,
-                                                       (float)(Tc[2] - Tc[3]), (float)(Tc[1] - Tc[3]),
-                                                       (float)(Tc[0] - Tc[3]), eps);
-    const double igm = 1.0 / (g.gam - 1.0);
-    Wl[4] = Wl[0] * g.R * Tl * igm + 0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
-    Wr[4] = Wr[0] * g.R * Tr * igm + 0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+    for (int s = 0; s < 6; s++) Tc[s] = temp_gr<double>(S[s], gR);
+    float l, r;
+    weno_pair<float>((float)(Tc[0] - Tc[2]), (float)(Tc[1] - Tc[2]), 0.f, (float)(Tc[3] - Tc[2]),
+                     (float)(Tc[4] - Tc[2]), (float)(Tc[5] - Tc[2]), eps, l, r);
+    const double Tl = Tc[2] + (double)l, Tr = Tc[2] + (double)r;
+    const double Rg = g.R / (g.gam - 1.0);
+    Wl[4] = Wl[0] * Rg * Tl + 0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
+    Wr[4] = Wr[0] * Rg * Tr + 0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
 }
 
 template <typename T>

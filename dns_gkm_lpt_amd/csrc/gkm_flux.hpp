@@ -307,58 +307,109 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         TS hu[7];
         half_tab<7>(st.u, st.th, (TS)h[0], (TS)h[1], hu);
 
-        TS sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0}, sB[5] = {0, 0, 0, 0, 0}, sF[5] = {0, 0, 0, 0, 0};
-        // equilibrium tangential slopes' right-hand sides (GKMflux.cpp:130-133, 141-144)
-        mom_u(hu, st.th, kp2, kp4, ay, sy);
-        mom_u(hu, st.th, kp2, kp4, az, sz);
-        // den_e B_ terms (GKMflux.cpp:183-190)
-        {
-            const TS cB = gm45 * den;
-            TS c[5];
+        if constexpr (sizeof(TS) == 8) {
+            // FP64: each term goes through the shift into E's tangential rest
+            // frame and straight into the accumulators (the shift is linear),
+            // so no per-side sums stay live: fewer registers spilled.
+            const TS dV = (TS)(S.v - E.v), dW = (TS)(S.w - E.w);
+            auto put = [&](TS t[5], TS acc[5]) {
+                tshift(t, dV, dW);
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cB * ax[q];
-            mom_u(hu + 1, st.th, kp2, kp4, c, sB);
+                for (int q = 0; q < 5; q++) acc[q] += t[q];
+            };
+            {   // equilibrium tangential slopes' right-hand sides (GKMflux.cpp:130-133, 141-144)
+                TS t[5] = {0, 0, 0, 0, 0};
+                mom_u(hu, st.th, kp2, kp4, ay, t);
+                put(t, bye);
+                TS u[5] = {0, 0, 0, 0, 0};
+                mom_u(hu, st.th, kp2, kp4, az, u);
+                put(u, bze);
+            }
+            {   // den_e B_ terms (GKMflux.cpp:183-190) and flux terms (GKMflux.cpp:223-228), by input
+                const TS cB = gm45 * den, cF = tp4 * den;
+                TS b[5] = {0, 0, 0, 0, 0}, f[5] = {0, 0, 0, 0, 0}, c[5];
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cB * ay[q];
-            mom_uvw(hu, st.th, kp4, c, sB, 2);
+                for (int q = 0; q < 5; q++) c[q] = cB * ax[q];
+                mom_u(hu + 1, st.th, kp2, kp4, c, b);
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cB * az[q];
-            mom_uvw(hu, st.th, kp4, c, sB, 3);
-            const TS cA = tgm5 * den;
+                for (int q = 0; q < 5; q++) c[q] = cF * ax[q];
+                mom_u(hu + 2, st.th, kp2, kp4, c, f);
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cB * tvw[q] + cA * A[q];
-            mom_u(hu, st.th, kp2, kp4, c, sB);
-        }
-        // flux terms (GKMflux.cpp:223-228)
-        {
-            const TS cF = tp4 * den;
-            TS c[5];
+                for (int q = 0; q < 5; q++) c[q] = cB * ay[q];
+                mom_uvw(hu, st.th, kp4, c, b, 2);
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cF * ax[q];
-            mom_u(hu + 2, st.th, kp2, kp4, c, sF);
+                for (int q = 0; q < 5; q++) c[q] = cF * ay[q];
+                mom_uvw(hu + 1, st.th, kp4, c, f, 2);
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cF * ay[q];
-            mom_uvw(hu + 1, st.th, kp4, c, sF, 2);
+                for (int q = 0; q < 5; q++) c[q] = cB * az[q];
+                mom_uvw(hu, st.th, kp4, c, b, 3);
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cF * az[q];
-            mom_uvw(hu + 1, st.th, kp4, c, sF, 3);
-            const TS cA = tp5 * den;
+                for (int q = 0; q < 5; q++) c[q] = cF * az[q];
+                mom_uvw(hu + 1, st.th, kp4, c, f, 3);
+                const TS cA = tgm5 * den, cA2 = tp5 * den;
 #pragma unroll
-            for (int q = 0; q < 5; q++) c[q] = cF * tvw[q] + cA * A[q];
-            mom_u(hu + 1, st.th, kp2, kp4, c, sF);
-        }
-        // into E's tangential rest frame
-        const TS dV = (TS)(S.v - E.v), dW = (TS)(S.w - E.w);
-        tshift(sy, dV, dW);
-        tshift(sz, dV, dW);
-        tshift(sB, dV, dW);
-        tshift(sF, dV, dW);
+                for (int q = 0; q < 5; q++) c[q] = cB * tvw[q] + cA * A[q];
+                mom_u(hu, st.th, kp2, kp4, c, b);
 #pragma unroll
-        for (int q = 0; q < 5; q++) {
-            bye[q] += sy[q];
-            bze[q] += sz[q];
-            Be[q] += sB[q];
-            Fs[q] += sF[q];
+                for (int q = 0; q < 5; q++) c[q] = cF * tvw[q] + cA2 * A[q];
+                mom_u(hu + 1, st.th, kp2, kp4, c, f);
+                put(b, Be);
+                put(f, Fs);
+            }
+        } else {
+            TS sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0}, sB[5] = {0, 0, 0, 0, 0}, sF[5] = {0, 0, 0, 0, 0};
+            // equilibrium tangential slopes' right-hand sides (GKMflux.cpp:130-133, 141-144)
+            mom_u(hu, st.th, kp2, kp4, ay, sy);
+            mom_u(hu, st.th, kp2, kp4, az, sz);
+            // den_e B_ terms (GKMflux.cpp:183-190)
+            {
+                const TS cB = gm45 * den;
+                TS c[5];
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cB * ax[q];
+                mom_u(hu + 1, st.th, kp2, kp4, c, sB);
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cB * ay[q];
+                mom_uvw(hu, st.th, kp4, c, sB, 2);
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cB * az[q];
+                mom_uvw(hu, st.th, kp4, c, sB, 3);
+                const TS cA = tgm5 * den;
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cB * tvw[q] + cA * A[q];
+                mom_u(hu, st.th, kp2, kp4, c, sB);
+            }
+            // flux terms (GKMflux.cpp:223-228)
+            {
+                const TS cF = tp4 * den;
+                TS c[5];
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cF * ax[q];
+                mom_u(hu + 2, st.th, kp2, kp4, c, sF);
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cF * ay[q];
+                mom_uvw(hu + 1, st.th, kp4, c, sF, 2);
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cF * az[q];
+                mom_uvw(hu + 1, st.th, kp4, c, sF, 3);
+                const TS cA = tp5 * den;
+#pragma unroll
+                for (int q = 0; q < 5; q++) c[q] = cF * tvw[q] + cA * A[q];
+                mom_u(hu + 1, st.th, kp2, kp4, c, sF);
+            }
+            // into E's tangential rest frame
+            const TS dV = (TS)(S.v - E.v), dW = (TS)(S.w - E.w);
+            tshift(sy, dV, dW);
+            tshift(sz, dV, dW);
+            tshift(sB, dV, dW);
+            tshift(sF, dV, dW);
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                bye[q] += sy[q];
+                bze[q] += sz[q];
+                Be[q] += sB[q];
+                Fs[q] += sF[q];
+            }
         }
     }
 

@@ -48,22 +48,22 @@ ALG_OPS_PER_CELL_UPDATE = 26951
 # profiles/sweep_traffic_512.md.  Used for roofline.traffic when the dominant
 # kernel of the run is one of these configurations.
 MEASURED_TRAFFIC_GIB = {
-    ("mixed", 0, 2, 512): (10.87, 5.00),
-    ("mixed", 1, 2, 512): (12.56, 5.00),
-    ("mixed", 2, 0, 512): (20.86, 5.00),
-    ("fp64", 0, 0, 512): (59.05, 29.28),
-    ("fp64", 1, 2, 512): (70.76, 27.62),
-    ("fp64", 2, 0, 512): (75.65, 29.16),
+    ("mixed", 0, 2, 512): (11.24, 5.00),
+    ("mixed", 1, 2, 512): (12.97, 5.00),
+    ("mixed", 2, 0, 512): (21.26, 5.00),
+    ("fp64", 0, 2, 512): (58.84, 24.08),
+    ("fp64", 1, 2, 512): (53.60, 21.58),
+    ("fp64", 2, 0, 512): (60.67, 21.86),
 }
 # rocprofv3 VALUBusy (% of time the VALU issues) of each sweep launch, keyed
 # like MEASURED_TRAFFIC_GIB; commands and table in profiles/sweep_valu_512.md.
 MEASURED_VALU_BUSY = {
-    ("mixed", 0, 2, 512): 83.9,
-    ("mixed", 1, 2, 512): 75.7,
-    ("mixed", 2, 0, 512): 87.2,
-    ("fp64", 0, 0, 512): 58.1,
-    ("fp64", 1, 2, 512): 52.3,
-    ("fp64", 2, 0, 512): 54.9,
+    ("mixed", 0, 2, 512): 80.3,
+    ("mixed", 1, 2, 512): 72.2,
+    ("mixed", 2, 0, 512): 79.8,
+    ("fp64", 0, 2, 512): 57.1,
+    ("fp64", 1, 2, 512): 50.9,
+    ("fp64", 2, 0, 512): 53.9,
 }
 KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
                 3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",

@@ -32,7 +32,7 @@ constexpr int kTileVariants = 3;
 constexpr int kTiles[3][kTileVariants][3] = {
     {{4, 4, 16}, {1, 16, 16}, {2, 8, 16}},
     {{4, 4, 16}, {1, 16, 16}, {2, 8, 16}},
-    {{16, 4, 4}, {4, 8, 8}, {1, 16, 16}},
+    {{16, 4, 4}, {4, 8, 8}, {8, 8, 4}},
 };
 template <int DIR, int V>
 struct Tile {

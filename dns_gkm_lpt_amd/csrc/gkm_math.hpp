@@ -10,13 +10,8 @@
 //                      1/20, 9/20, 9/20, 1/20, IS3 := max IS, alpha = d/(eps+IS),
 //                      energy reconstructed through temperature)
 //   collision time     update.cpp:276-278 (Sutherland, left cell of the face)
-//   GKS flux           GKMflux.cpp:11-229 with MCal GKMflux.cpp:238-278 and
-//                      slopesolver GKMflux.cpp:283-297 (a[4] scaled by 1/Pr)
-// The flux is reorganised by Maxwellian state: the interface terms that belong
-// to the left state, the right state and the equilibrium state are each summed
-// in one block while that state's moment tables are live, and the collision
-// coefficients (which need only the densities and lambdas) are computed first.
-// This changes the order of the floating-point sums, not the terms.
+//   Maxwellian moment tables (GKMflux.cpp:40-68), used by the GKS flux in
+//   gkm_flux.hpp.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -229,259 +224,14 @@ __host__ __device__ __forceinline__ void half_tab(T m, T i2l, T h0, T h1, T* h)
 }
 
 // <u^k v^l w^m psi>, psi = (1, u, v, w, (u^2+v^2+w^2+xi^2)/2)  (GKMflux.cpp:269-278)
-template <int k, int l, int m, typename T>
-__host__ __device__ __forceinline__ void mom(const T* tu, const T* tv, const T* tw, T e2, T M[5])
-{
-    T t = tu[k] * tv[l] * tw[m];
-    M[0] = t;
-    M[1] = tu[k + 1] * tv[l] * tw[m];
-    M[2] = tu[k] * tv[l + 1] * tw[m];
-    M[3] = tu[k] * tv[l] * tw[m + 1];
-    M[4] = T(0.5) * (tu[k + 2] * tv[l] * tw[m] + tu[k] * tv[l + 2] * tw[m] + tu[k] * tv[l] * tw[m + 2] + t * e2);
-}
-
-// acc += c * <u^k v^l w^m psi (a . psi)>  (GKMflux.cpp:238-267)
-template <int k, int l, int m, typename T>
-__host__ __device__ __forceinline__ void mom_a(const T* tu, const T* tv, const T* tw, T e2, T e4, const T a[5], T c,
-                                               T acc[5])
-{
-#define T3(p, q, r) (tu[p] * tv[q] * tw[r])
-    T h0 = T(0.5) * (T3(2 + k, l, m) + T3(k, 2 + l, m) + T3(k, l, 2 + m) + T3(k, l, m) * e2);
-    T h1 = T(0.5) * (T3(3 + k, l, m) + T3(1 + k, 2 + l, m) + T3(1 + k, l, 2 + m) + T3(1 + k, l, m) * e2);
-    T h2 = T(0.5) * (T3(2 + k, 1 + l, m) + T3(k, 3 + l, m) + T3(k, 1 + l, 2 + m) + T3(k, 1 + l, m) * e2);
-    T h3 = T(0.5) * (T3(2 + k, l, 1 + m) + T3(k, 2 + l, 1 + m) + T3(k, l, 3 + m) + T3(k, l, 1 + m) * e2);
-    T m0 = a[0] * T3(k, l, m) + a[1] * T3(1 + k, l, m) + a[2] * T3(k, 1 + l, m) + a[3] * T3(k, l, 1 + m) + a[4] * h0;
-    T m1 = a[0] * T3(1 + k, l, m) + a[1] * T3(2 + k, l, m) + a[2] * T3(1 + k, 1 + l, m) + a[3] * T3(1 + k, l, 1 + m) +
-           a[4] * h1;
-    T m2 = a[0] * T3(k, 1 + l, m) + a[1] * T3(1 + k, 1 + l, m) + a[2] * T3(k, 2 + l, m) + a[3] * T3(k, 1 + l, 1 + m) +
-           a[4] * h2;
-    T m3 = a[0] * T3(k, l, 1 + m) + a[1] * T3(1 + k, l, 1 + m) + a[2] * T3(k, 1 + l, 1 + m) + a[3] * T3(k, l, 2 + m) +
-           a[4] * h3;
-    T m4 = T(0.25) * a[4] *
-               (T3(4 + k, l, m) + T3(k, 4 + l, m) + T3(k, l, 4 + m) + T3(k, l, m) * e4 +
-                T(2) * (T3(2 + k, 2 + l, m) + T3(2 + k, l, 2 + m) + T3(k, 2 + l, 2 + m)) +
-                T(2) * e2 * (T3(2 + k, l, m) + T3(k, 2 + l, m) + T3(k, l, 2 + m))) +
-           a[0] * h0 + a[1] * h1 + a[2] * h2 + a[3] * h3;
-#undef T3
-    acc[0] += c * m0;
-    acc[1] += c * m1;
-    acc[2] += c * m2;
-    acc[3] += c * m3;
-    acc[4] += c * m4;
-}
-
-// micro slope a from macro slope b (GKMflux.cpp:283-297), a[4] scaled by 1/Pr
-template <typename T>
-__host__ __device__ __forceinline__ void slope(const T b[5], const Maxw<T>& s, const Gas<T>& g, T a[5])
-{
-    T R2 = b[1] - s.u * b[0];
-    T R3 = b[2] - s.v * b[0];
-    T R4 = b[3] - s.w * b[0];
-    T q = s.u * s.u + s.v * s.v + s.w * s.w + T(g.K + 3) * s.i2l;
-    T R5 = T(2) * b[4] - b[0] * q;
-    T lam2 = T(2) * s.lam;
-    // (4 lambda^2)/(K+3) = lam2^2 * 1/(K+3); K+3 is 5 for gamma = 1.4
-    a[4] = g.inv_prn * (R5 - T(2) * (s.u * R2 + s.v * R3 + s.w * R4)) * (lam2 * lam2) * (T(1) / T(g.K + 3));
-    a[3] = lam2 * R4 - s.w * a[4];
-    a[2] = lam2 * R3 - s.v * a[4];
-    a[1] = lam2 * R2 - s.u * a[4];
-    a[0] = b[0] - a[1] * s.u - a[2] * s.v - a[3] * s.w - T(0.5) * a[4] * q;
-}
-
-// ---------------------------------------------------------------------------
-// GKS time-integrated interface flux (GKMflux.cpp:11-229).
-//   WL, WR : cell states either side of the face (rotated frame)
-//   Wl, Wr : WENO states at the face
-//   D[0..5]: DWxl, DWxr, DWyl, DWyr, DWzl, DWzr
-//   tau    : collision time of the left cell (the pressure-jump term is added here)
-// ---------------------------------------------------------------------------
-template <typename T>
-__host__ __device__ __forceinline__ void gks_flux(const T WL[5], const T WR[5], const T Wl[5], const T Wr[5],
-                                                  const T D[6][5], T tau, T dt, T dx, const Gas<T>& g, T F[5])
-{
-    const T pi = T(3.141592653589793238462643383279502884);
-    Maxw<T> L, R;
-    maxw_init(L, Wl, g);
-    maxw_init(R, Wr, g);
-
-    // collision time with the artificial pressure-jump term (GKMflux.cpp:156-157)
-    {
-        T pl = L.den / L.lam, pr = R.den / R.lam;
-        tau = tau + dt * (fabs(pl - pr) / fabs(pl + pr));
-    }
-    // time-integration coefficients (GKMflux.cpp:162-167, 197-204)
-    const T ex = g_exp(-dt / tau);
-    const T om = T(1) - ex;
-    const T igm0 = T(1) / (dt - tau * om);
-    const T gm1 = -om * igm0;
-    const T gm2 = (-dt + T(2) * tau * om - dt * ex) * igm0;
-    const T gm3 = -gm1;
-    const T gm4 = (dt * ex - tau * om) * igm0;
-    const T gm5 = tau * gm3;
-    const T Pc = tau * om;
-    const T Qc = -tau * dt * ex + tau * tau * om;
-    const T p0 = dt - Pc;
-    const T p1 = T(0.5) * dt * dt - tau * p0;
-    const T p2 = -tau * p0 + Qc;
-    const T p3 = Pc;
-    const T p4 = -Qc - tau * Pc;
-    const T p5 = -tau * Pc;
-
-    // error-function / Gaussian terms of the half-space tables
-    const T gl = T(0.5) * g_exp(-L.lam * L.u * L.u) / g_sqrt(L.lam * pi);
-    const T hl0 = T(0.5) * g_erfc(-g_sqrt(L.lam) * L.u);
-    const T gr = T(0.5) * g_exp(-R.lam * R.u * R.u) / g_sqrt(R.lam * pi);
-    const T hr0 = T(0.5) * g_erfc(g_sqrt(R.lam) * R.u);
-
-    // equilibrium state at the face (GKMflux.cpp:73-80)
-    T We[5];
-    {
-        T hu[3], tv[3], tw[3], M[5];
-        half_tab<3>(L.u, L.i2l, hl0, L.u * hl0 + gl, hu);
-        full_tab<3>(L.v, L.i2l, tv);
-        full_tab<3>(L.w, L.i2l, tw);
-        mom<0, 0, 0>(hu, tv, tw, L.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) We[q] = L.den * M[q];
-        half_tab<3>(R.u, R.i2l, hr0, R.u * hr0 - gr, hu);
-        full_tab<3>(R.v, R.i2l, tv);
-        full_tab<3>(R.w, R.i2l, tw);
-        mom<0, 0, 0>(hu, tv, tw, R.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) We[q] += R.den * M[q];
-    }
-    Maxw<T> E;
-    maxw_init(E, We, g);
-
-    // Per-state sums.  Be accumulates den_e * B_ (GKMflux.cpp:183-190),
-    // F accumulates the flux (GKMflux.cpp:223-228); bye/bze are the
-    // phase-II right-hand sides of the equilibrium slopes (GKMflux.cpp:141-144).
-    T Be[5] = {0, 0, 0, 0, 0}, bye[5] = {0, 0, 0, 0, 0}, bze[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 5; q++) F[q] = T(0);
-
-    // ---- left (u>0 half of the left Maxwellian) and right (u<0 half of the right).
-    // Each table set is live only inside its block: the full u-table gives the
-    // time slope A of the state first, then the half table carries every term
-    // that integrates over the half space.
-#pragma unroll
-    for (int side = 0; side < 2; side++) {
-        const Maxw<T>& S = side == 0 ? L : R;
-        const int dxi = side == 0 ? 0 : 1;  // index of DWx{l,r} etc. in D
-        T tv[6], tw[6];
-        full_tab<6>(S.v, S.i2l, tv);
-        full_tab<6>(S.w, S.i2l, tw);
-        const T iden = T(1) / S.den;
-        T b[5], ax[5], ay[5], az[5], A[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = D[0 + dxi][q] * iden;
-        slope(b, S, g, ax);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = D[2 + dxi][q] * iden;
-        slope(b, S, g, ay);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = D[4 + dxi][q] * iden;
-        slope(b, S, g, az);
-        {   // phase II: A from the full-space moments (GKMflux.cpp:134-151)
-            T fu[6];
-            full_tab<6>(S.u, S.i2l, fu);
-#pragma unroll
-            for (int q = 0; q < 5; q++) b[q] = T(0);
-            mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, T(-1), b);
-            mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, T(-1), b);
-            mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, T(-1), b);
-            slope(b, S, g, A);
-        }
-        T hu[7];
-        if (side == 0)
-            half_tab<7>(S.u, S.i2l, hl0, S.u * hl0 + gl, hu);
-        else
-            half_tab<7>(S.u, S.i2l, hr0, S.u * hr0 - gr, hu);
-        // phase II right-hand sides of the equilibrium slopes (GKMflux.cpp:130-133, 141-144)
-        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, ay, T(1), bye);
-        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, az, T(1), bze);
-        // this state's share of den_e * B_ and of F
-        const T cB = (gm4 + gm5) * S.den;
-        T M[5];
-        mom<0, 0, 0>(hu, tv, tw, S.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) Be[q] += gm3 * S.den * M[q];
-        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cB, Be);
-        mom_a<0, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cB, Be);
-        mom_a<0, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cB, Be);
-        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, A, gm5 * S.den, Be);
-        mom<1, 0, 0>(hu, tv, tw, S.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) F[q] += p3 * S.den * M[q];
-        const T cF = p4 * S.den;
-        mom_a<2, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cF, F);
-        mom_a<1, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cF, F);
-        mom_a<1, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cF, F);
-        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, A, p5 * S.den, F);
-    }
-
-    // ---- equilibrium state: one half-space table at a time, then the full table
-    {
-        T tv[6], tw[6];
-        full_tab<6>(E.v, E.i2l, tv);
-        full_tab<6>(E.w, E.i2l, tw);
-        const T ge = T(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
-        const T sl = g_sqrt(E.lam) * E.u;
-        const T c = T(2) / (E.den * dx);
-        const T cB = gm2 * E.den;
-        const T cF = p2 * E.den;
-        T b[5], a[5];
-        {   // u>0 half with the left normal slope (GKMflux.cpp:115, 124, 170, 210)
-            T h[7];
-            const T h0 = T(0.5) * g_erfc(-sl);
-            half_tab<7>(E.u, E.i2l, h0, E.u * h0 + ge, h);
-#pragma unroll
-            for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * c;
-            slope(b, E, g, a);
-            mom_a<1, 0, 0>(h, tv, tw, E.e2, E.e4, a, cB, Be);
-            mom_a<2, 0, 0>(h, tv, tw, E.e2, E.e4, a, cF, F);
-        }
-        {   // u<0 half with the right normal slope (GKMflux.cpp:116, 125, 171, 211)
-            T h[7];
-            const T h0 = T(0.5) * g_erfc(sl);
-            half_tab<7>(E.u, E.i2l, h0, E.u * h0 - ge, h);
-#pragma unroll
-            for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * c;
-            slope(b, E, g, a);
-            mom_a<1, 0, 0>(h, tv, tw, E.e2, E.e4, a, cB, Be);
-            mom_a<2, 0, 0>(h, tv, tw, E.e2, E.e4, a, cF, F);
-        }
-        T fu[7];
-        full_tab<7>(E.u, E.i2l, fu);
-        // tangential slopes (GKMflux.cpp:148-149, 172-173, 212-213)
-        slope(bye, E, g, a);
-        mom_a<0, 1, 0>(fu, tv, tw, E.e2, E.e4, a, cB, Be);
-        mom_a<1, 1, 0>(fu, tv, tw, E.e2, E.e4, a, cF, F);
-        slope(bze, E, g, a);
-        mom_a<0, 0, 1>(fu, tv, tw, E.e2, E.e4, a, cB, Be);
-        mom_a<1, 0, 1>(fu, tv, tw, E.e2, E.e4, a, cF, F);
-        // B_ and the time slope A_ (GKMflux.cpp:183-191)
-        T M[5];
-        mom<0, 0, 0>(fu, tv, tw, E.e2, M);
-        const T iden = T(1) / E.den;
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (Be[q] + gm1 * E.den * M[q]) * iden;
-        slope(b, E, g, a);
-        // equilibrium part of the flux (GKMflux.cpp:225)
-        mom<1, 0, 0>(fu, tv, tw, E.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) F[q] += p0 * E.den * M[q];
-        mom_a<1, 0, 0>(fu, tv, tw, E.e2, E.e4, a, p1 * E.den, F);
-    }
-}
-
 // ===========================================================================
 // Mixed precision.  FP64 where cancellation would cost digits: the stencil
 // differences fed to WENO, primitive recovery (c2p, lambda = rho/(2p)), the
 // equilibrium state We, the Euler part of the flux (the p0 and p3 terms,
 // GKMflux.cpp:225-226, which carry dt times the inviscid flux) and the
 // collision coefficients.  FP32 for WENO smoothness indicators and weights
-// and for every slope-dependent moment term of the flux.
+// and for every slope-dependent moment term of the flux (the flux's split is
+// gks_flux_tf<double, float>, gkm_flux.hpp).
 // ===========================================================================
 
 // WENO on FP64 stencils, FP32 arithmetic on differences from cell i, the base
@@ -513,192 +263,9 @@ __host__ __device__ __forceinline__ void weno5_mixed(const double S[6][5], const
     Wr[4] = Wr[0] * Rg * Tr + 0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
 }
 
-template <typename T>
-__host__ __device__ __forceinline__ Maxw<float> to_f32(const Maxw<T>& s)
-{
-    return Maxw<float>{(float)s.den, (float)s.u, (float)s.v, (float)s.w, (float)s.lam, (float)s.i2l, (float)s.e2,
-                       (float)s.e4};
-}
-
 __host__ __device__ __forceinline__ Gas<float> to_f32(const Gas<double>& g)
 {
     return Gas<float>{(float)g.gam, (float)g.R, (float)g.inv_prn, (float)g.mu0, (float)g.T0, (float)g.eps, g.K};
-}
-
-// Same inputs and result as gks_flux<double>; D (the ten slopes) arrives in FP64.
-__host__ __device__ __forceinline__ void gks_flux_mixed(const double WL[5], const double WR[5], const double Wl[5],
-                                                        const double Wr[5], const double D[6][5], double tau,
-                                                        double dt, double dx, const Gas<double>& g, double F[5])
-{
-    const double pi = 3.141592653589793238462643383279502884;
-    const Gas<float> gf = to_f32(g);
-    Maxw<double> L, R;
-    maxw_init(L, Wl, g);
-    maxw_init(R, Wr, g);
-    {
-        const double pl = L.den / L.lam, pr = R.den / R.lam;
-        tau = tau + dt * (fabs(pl - pr) / fabs(pl + pr));
-    }
-    const double ex = exp(-dt / tau);
-    const double om = 1.0 - ex;
-    const double igm0 = 1.0 / (dt - tau * om);
-    const double gm1 = -om * igm0;
-    const double gm2 = (-dt + 2.0 * tau * om - dt * ex) * igm0;
-    const double gm3 = -gm1;
-    const double gm4 = (dt * ex - tau * om) * igm0;
-    const double gm5 = tau * gm3;
-    const double Pc = tau * om;
-    const double Qc = -tau * dt * ex + tau * tau * om;
-    const double p0 = dt - Pc;
-    const double p1 = 0.5 * dt * dt - tau * p0;
-    const double p2 = -tau * p0 + Qc;
-    const double p3 = Pc;
-    const double p4 = -Qc - tau * Pc;
-    const double p5 = -tau * Pc;
-
-    const double gl = 0.5 * exp(-L.lam * L.u * L.u) / sqrt(L.lam * pi);
-    const double hl0 = 0.5 * erfc(-sqrt(L.lam) * L.u);
-    const double gr = 0.5 * exp(-R.lam * R.u * R.u) / sqrt(R.lam * pi);
-    const double hr0 = 0.5 * erfc(sqrt(R.lam) * R.u);
-
-    // FP64: equilibrium state, the Euler part p3 (l, r) and the 000 moments
-    double We[5], F64[5], Bz[5];
-    {
-        double hu[4], tv[3], tw[3], M[5];
-        half_tab<4>(L.u, L.i2l, hl0, L.u * hl0 + gl, hu);
-        full_tab<3>(L.v, L.i2l, tv);
-        full_tab<3>(L.w, L.i2l, tw);
-        mom<0, 0, 0>(hu, tv, tw, L.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) We[q] = L.den * M[q];
-        mom<1, 0, 0>(hu, tv, tw, L.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) F64[q] = p3 * L.den * M[q];
-        half_tab<4>(R.u, R.i2l, hr0, R.u * hr0 - gr, hu);
-        full_tab<3>(R.v, R.i2l, tv);
-        full_tab<3>(R.w, R.i2l, tw);
-        mom<0, 0, 0>(hu, tv, tw, R.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) We[q] += R.den * M[q];
-        mom<1, 0, 0>(hu, tv, tw, R.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) F64[q] += p3 * R.den * M[q];
-    }
-    Maxw<double> E;
-    maxw_init(E, We, g);
-    {
-        double fu[4], tv[3], tw[3], M[5];
-        full_tab<4>(E.u, E.i2l, fu);
-        full_tab<3>(E.v, E.i2l, tv);
-        full_tab<3>(E.w, E.i2l, tw);
-        mom<1, 0, 0>(fu, tv, tw, E.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) F64[q] += p0 * E.den * M[q];
-        // gm1 den_e <psi>_e + gm3 We (GKMflux.cpp:185): zero up to rounding
-        mom<0, 0, 0>(fu, tv, tw, E.e2, M);
-#pragma unroll
-        for (int q = 0; q < 5; q++) Bz[q] = gm1 * E.den * M[q] + gm3 * We[q];
-    }
-
-    // FP32: every slope-dependent term, state by state as in gks_flux
-    const float fgm2 = (float)gm2, fgm45 = (float)(gm4 + gm5), fgm5 = (float)gm5;
-    const float fp1 = (float)p1, fp2 = (float)p2, fp4 = (float)p4, fp5 = (float)p5;
-    float Be[5], bye[5] = {0, 0, 0, 0, 0}, bze[5] = {0, 0, 0, 0, 0}, Ff[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 5; q++) Be[q] = (float)Bz[q];
-#pragma unroll
-    for (int side = 0; side < 2; side++) {
-        const Maxw<float> S = to_f32(side == 0 ? L : R);
-        const int dxi = side;
-        float tv[6], tw[6];
-        full_tab<6>(S.v, S.i2l, tv);
-        full_tab<6>(S.w, S.i2l, tw);
-        const double iden = 1.0 / (side == 0 ? L.den : R.den);
-        float b[5], ax[5], ay[5], az[5], A[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (float)(D[0 + dxi][q] * iden);
-        slope(b, S, gf, ax);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (float)(D[2 + dxi][q] * iden);
-        slope(b, S, gf, ay);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = (float)(D[4 + dxi][q] * iden);
-        slope(b, S, gf, az);
-        {
-            float fu[6];
-            full_tab<6>(S.u, S.i2l, fu);
-#pragma unroll
-            for (int q = 0; q < 5; q++) b[q] = 0.f;
-            mom_a<1, 0, 0>(fu, tv, tw, S.e2, S.e4, ax, -1.f, b);
-            mom_a<0, 1, 0>(fu, tv, tw, S.e2, S.e4, ay, -1.f, b);
-            mom_a<0, 0, 1>(fu, tv, tw, S.e2, S.e4, az, -1.f, b);
-            slope(b, S, gf, A);
-        }
-        float hu[7];
-        if (side == 0)
-            half_tab<7>(S.u, S.i2l, (float)hl0, (float)(L.u * hl0 + gl), hu);
-        else
-            half_tab<7>(S.u, S.i2l, (float)hr0, (float)(R.u * hr0 - gr), hu);
-        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, ay, 1.f, bye);
-        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, az, 1.f, bze);
-        const float cB = fgm45 * S.den;
-        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cB, Be);
-        mom_a<0, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cB, Be);
-        mom_a<0, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cB, Be);
-        mom_a<0, 0, 0>(hu, tv, tw, S.e2, S.e4, A, fgm5 * S.den, Be);
-        const float cF = fp4 * S.den;
-        mom_a<2, 0, 0>(hu, tv, tw, S.e2, S.e4, ax, cF, Ff);
-        mom_a<1, 1, 0>(hu, tv, tw, S.e2, S.e4, ay, cF, Ff);
-        mom_a<1, 0, 1>(hu, tv, tw, S.e2, S.e4, az, cF, Ff);
-        mom_a<1, 0, 0>(hu, tv, tw, S.e2, S.e4, A, fp5 * S.den, Ff);
-    }
-    {
-        const Maxw<float> Ef = to_f32(E);
-        const double ge = 0.5 * exp(-E.lam * E.u * E.u) / sqrt(E.lam * pi);
-        const double sl = sqrt(E.lam) * E.u;
-        float tv[6], tw[6];
-        full_tab<6>(Ef.v, Ef.i2l, tv);
-        full_tab<6>(Ef.w, Ef.i2l, tw);
-        const double c = 2.0 / (E.den * dx);
-        const float cB = fgm2 * Ef.den;
-        const float cF = fp2 * Ef.den;
-        float b[5], a[5];
-        {
-            float h[7];
-            const double h0 = 0.5 * erfc(-sl);
-            half_tab<7>(Ef.u, Ef.i2l, (float)h0, (float)(E.u * h0 + ge), h);
-#pragma unroll
-            for (int q = 0; q < 5; q++) b[q] = (float)((We[q] - WL[q]) * c);
-            slope(b, Ef, gf, a);
-            mom_a<1, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
-            mom_a<2, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
-        }
-        {
-            float h[7];
-            const double h0 = 0.5 * erfc(sl);
-            half_tab<7>(Ef.u, Ef.i2l, (float)h0, (float)(E.u * h0 - ge), h);
-#pragma unroll
-            for (int q = 0; q < 5; q++) b[q] = (float)((WR[q] - We[q]) * c);
-            slope(b, Ef, gf, a);
-            mom_a<1, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
-            mom_a<2, 0, 0>(h, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
-        }
-        float fu[7];
-        full_tab<7>(Ef.u, Ef.i2l, fu);
-        slope(bye, Ef, gf, a);
-        mom_a<0, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
-        mom_a<1, 1, 0>(fu, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
-        slope(bze, Ef, gf, a);
-        mom_a<0, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, a, cB, Be);
-        mom_a<1, 0, 1>(fu, tv, tw, Ef.e2, Ef.e4, a, cF, Ff);
-        const float iden = (float)(1.0 / E.den);
-#pragma unroll
-        for (int q = 0; q < 5; q++) Be[q] *= iden;
-        slope(Be, Ef, gf, a);
-        mom_a<1, 0, 0>(fu, tv, tw, Ef.e2, Ef.e4, a, fp1 * Ef.den, Ff);
-    }
-#pragma unroll
-    for (int q = 0; q < 5; q++) F[q] = F64[q] + (double)Ff[q];
 }
 
 }  // namespace gkm

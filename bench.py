@@ -1,7 +1,8 @@
 """Benchmark of the GKM-WENO step and the particle tracker on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nc 512] [--precision both|mixed|fp64]
-                    [--particles N] [--tiles auto|x,y,z] [--no-cpu-baseline]
+                    [--particles N] [--tiles auto|x,y,z] [--ensemble-steps N] [--adjoint-steps N]
+                    [--no-cpu-baseline]
 
 A step is one Eulerian GKM-WENO step (CFL dt on the device, x/y/z fused sweeps
 with the update) of the periodic cube, state resident in HBM.  The workload is
@@ -10,9 +11,10 @@ tracers; the FP64 step of the same state is timed beside it ("FP64 vs mixed").
 Prints one JSON line: cell-updates/s of the mixed step (`value`), the FP64
 figures, the HBM-roofline fraction of the dominant kernel (HIP events on the
 solver's stream), its VALU rate against the reference's operation count,
-particle-updates/s, and the CPU baseline (the reference's own evolveEulerian
-when oracle/_ref is built, else the oracle restatement) timed on a bounded
-sample on this host.
+particle-updates/s, the 64-member ensemble (per-phase graph, eager and
+persistent-kernel steps), the 64^3 discrete-adjoint gradient, and the CPU
+baseline (the reference's own evolveEulerian when oracle/_ref is built, else
+the oracle restatement) timed on a bounded sample on this host.
 """
 import argparse
 import json
@@ -239,6 +241,36 @@ def run_ensemble(gl, steps, warmup):
     return out
 
 
+def run_adjoint(steps):
+    """The discrete-adjoint configuration: dJ/du0 of the final kinetic energy
+    J = <rho |u|^2 / 2> with respect to the initial velocity, 64^3 periodic
+    cube (Mt 0.3, Re_lambda 40, k0 4, seed 0), FP64, dt fixed from the initial
+    CFL 0.5, checkpoints every 20 steps.  Timed separately: one gradient
+    (forward run keeping checkpoints + adjoint run recomputing each segment),
+    one tangent-linear run along a seeded band-limited unit direction, and one
+    forward run; the adjoint and tangent-linear directional derivatives are
+    compared."""
+    from dns_gkm_lpt_amd import adjoint as adj
+
+    pb, u = adj.make_problem(64, seed=0, nsteps=steps, every=20)
+    d = adj.band_limited_direction(u.shape, 100)
+    t = time.perf_counter()
+    J, g = pb.gradient(u)
+    s_grad = time.perf_counter() - t
+    t = time.perf_counter()
+    tl = pb.directional(u, d)
+    s_tl = time.perf_counter() - t
+    t = time.perf_counter()
+    pb.objective(u)
+    s_fwd = time.perf_counter() - t
+    ad = float(np.sum(g * d))
+    out = {"grid": "64^3", "steps": steps, "checkpoint_every": 20, "precision": "fp64", "dt": pb.dt,
+           "s_gradient": s_grad, "s_tangent_linear": s_tl, "s_forward": s_fwd,
+           "gradient_over_forward": s_grad / s_fwd, "adjoint_vs_tangent_linear": abs(ad - tl) / abs(tl)}
+    log(f"bench: adjoint {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +283,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 64-member ensemble configuration (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the 64^3 discrete-adjoint gradient configuration (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
     ap.add_argument("--tiles", default="auto",
                     help="sweep tile variants x,y,z (e.g. 0,0,0) or 'auto' to time them before the run")
@@ -337,6 +371,8 @@ def main():
         out["particles"] = particles
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(gl, args.ensemble_steps, args.warmup)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = run_adjoint(args.adjoint_steps)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

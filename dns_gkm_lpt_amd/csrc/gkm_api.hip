@@ -38,6 +38,7 @@ struct gkm_ctx {
     double* partial = nullptr;     // CFL block maxima
     bool have_state = false;
     int tile[3] = {0, 0, 0};       // sweep tile variant per direction
+    int slots = 512;               // concurrent sweep workgroups: 2 per CU
     std::string err;
     // per-kernel timing
     bool timing = false;
@@ -101,7 +102,7 @@ gkm::SweepArgs sweep_args(const gkm_ctx* c, int dir, int mode)
     a.inv_dx = 1.0 / c->dx;
     a.g = c->gas;
     a.tile = c->tile[dir];
-    a.seg = gkm::sweep_seg_len(dir, c->nc, a.tile);
+    a.seg = gkm::sweep_seg_len(dir, c->nc, a.tile, c->slots);
     a.nseg = c->nc / a.seg;
     a.mode = mode;
     return a;
@@ -164,6 +165,11 @@ int gkm_create(const gkm_config* cfg, gkm_ctx** out)
     if (e != hipSuccess) return bad(std::string("hipSetDevice: ") + hipGetErrorString(e));
     *out = c;
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        int cus = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device));
+        c->slots = 2 * (cus > 0 ? cus : 1);  // the sweep kernels run two workgroups per CU
+    }
     HIPCHK(c, hipEventCreate(&c->ev[0]));
     HIPCHK(c, hipEventCreate(&c->ev[1]));
     const size_t sbytes = sizeof(double) * 5 * (size_t)c->plane;
@@ -443,7 +449,7 @@ int gkm_autotune(gkm_ctx* c, int reps, int tile[3], double ms[9])
         for (int v = 0; v < gkm::kTileVariants; v++) {
             gkm::SweepArgs a = sweep_args(c, dir, dir);
             a.tile = v;
-            a.seg = gkm::sweep_seg_len(dir, c->nc, v);
+            a.seg = gkm::sweep_seg_len(dir, c->nc, v, c->slots);
             a.nseg = c->nc / a.seg;
             // dW is scratch and Wout is the next state's buffer: the resident state is not changed
             hipError_t e = gkm::launch_sweep(dir, a, mixed, c->stream);  // warm-up
@@ -956,7 +962,7 @@ gkm::SweepArgs ens_sweep_args(const gkm_ctx* c, int p, int d, int v)
     a.inv_dx = 1.0 / c->dx;
     a.g = c->gas;
     a.tile = v;
-    a.seg = gkm::sweep_seg_len(d, c->nc, v);
+    a.seg = gkm::sweep_seg_len(d, c->nc, v, 0);
     a.nseg = c->nc / a.seg;
     a.mode = d;
     return a;

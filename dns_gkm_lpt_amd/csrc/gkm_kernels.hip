@@ -45,14 +45,32 @@ bool sweep_grid_ok(int nc)
     return nc >= 32 && nc % 16 == 0;
 }
 
-int sweep_seg_len(int dir, int nc, int tile)
+int sweep_seg_len(int dir, int nc, int tile, int slots)
 {
-    // faces per workgroup: the whole line when that already gives >= 1024
-    // workgroups (4 per CU), otherwise split; each split costs one lead-in
-    // chunk, so segments stay >= 8 chunks and >= 64 faces long.
+    // faces per workgroup along the sweep axis; a segment costs seg/CN chunks
+    // plus one lead-in chunk (the face before the segment).
     if (dir < 0 || dir > 2 || tile < 0 || tile >= kTileVariants) return 0;
     const int CN = kTiles[dir][tile][0], CA = kTiles[dir][tile][1], CB = kTiles[dir][tile][2];
     const long pencils = (long)(nc / CA) * (nc / CB);
+    if (slots > 0) {
+        // one grid: the length with the fewest chunk-times, counting rounds of
+        // `slots` concurrent workgroups; the longer segment on ties
+        int best = nc;
+        long best_cost = -1;
+        for (int seg = nc; seg >= CN && seg % CN == 0 && nc % seg == 0; seg /= 2) {
+            const long blocks = pencils * (nc / seg);
+            const long cost = (blocks + slots - 1) / slots * (seg / CN + 1);
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                best = seg;
+            }
+            if (seg % 2) break;
+        }
+        return best;
+    }
+    // members of an ensemble (their workgroups add up): the whole line when
+    // that already gives >= 1024 workgroups (4 per CU), otherwise split while
+    // segments stay >= 8 chunks and >= 64 faces long.
     const int min_seg = 8 * CN > 64 ? 8 * CN : 64;
     int seg = nc;
     while (pencils * (nc / seg) < 1024 && seg % 2 == 0 && (seg / 2) % CN == 0 && seg / 2 >= min_seg) seg /= 2;

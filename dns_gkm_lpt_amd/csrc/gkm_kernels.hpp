@@ -46,7 +46,9 @@ int sweep_blocks(int dir, int nc, int tile, int seg);
 // blocks [first[m], first[m+1]) of nblk = first[nmem]; one tile variant for all
 hipError_t launch_ens_sweep(int dir, int tile, const SweepArgs* members, const int* first, int nmem, int nblk,
                             bool mixed, hipStream_t s);
-int sweep_seg_len(int dir, int nc, int tile);  // faces per workgroup chosen for a grid
+// faces per workgroup chosen for a grid: slots = concurrent workgroups of the
+// device for a single grid, 0 for the members of an ensemble
+int sweep_seg_len(int dir, int nc, int tile, int slots);
 bool sweep_grid_ok(int nc);                    // nc compatible with every tile shape
 
 // CFL: partial maxima (one per block) then a one-block final pass writing

@@ -53,9 +53,11 @@ MEASURED_TRAFFIC_GIB = {
     ("mixed", 0, 2, 512): (11.24, 5.00),
     ("mixed", 1, 2, 512): (12.97, 5.00),
     ("mixed", 2, 0, 512): (21.26, 5.00),
+    ("mixed", 2, 2, 512): (30.92, 5.20),
     ("fp64", 0, 2, 512): (58.84, 24.08),
     ("fp64", 1, 2, 512): (53.60, 21.58),
     ("fp64", 2, 0, 512): (60.67, 21.86),
+    ("fp64", 2, 2, 512): (74.41, 21.43),
 }
 # rocprofv3 VALUBusy (% of time the VALU issues) of each sweep launch, keyed
 # like MEASURED_TRAFFIC_GIB; commands and table in profiles/sweep_valu_512.md.
@@ -63,9 +65,11 @@ MEASURED_VALU_BUSY = {
     ("mixed", 0, 2, 512): 80.3,
     ("mixed", 1, 2, 512): 72.2,
     ("mixed", 2, 0, 512): 79.8,
+    ("mixed", 2, 2, 512): 75.3,
     ("fp64", 0, 2, 512): 57.1,
     ("fp64", 1, 2, 512): 50.9,
     ("fp64", 2, 0, 512): 53.9,
+    ("fp64", 2, 2, 512): 51.7,
 }
 KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
                 3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",

@@ -1,7 +1,7 @@
 """Benchmark of the GKM-WENO step and the particle tracker on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nc 512] [--precision both|mixed|fp64]
-                    [--particles N] [--tiles auto|x,y,z] [--ensemble-steps N] [--adjoint-steps N]
+                    [--particles N] [--tiles auto|x,y,z] [--ensemble-steps N] [--c2-steps N] [--adjoint-steps N]
                     [--no-cpu-baseline]
 
 A step is one Eulerian GKM-WENO step (CFL dt on the device, x/y/z fused sweeps
@@ -12,7 +12,8 @@ Prints one JSON line: cell-updates/s of the mixed step (`value`), the FP64
 figures, the HBM-roofline fraction of the dominant kernel (HIP events on the
 solver's stream), its VALU rate against the reference's operation count,
 particle-updates/s, the 64-member ensemble (per-phase graph, eager and
-persistent-kernel steps), the 64^3 discrete-adjoint gradient, and the CPU
+persistent-kernel steps), the 128^3 FP64 run with 2^18 tracers (500 coupled
+steps), the 64^3 discrete-adjoint gradient, and the CPU
 baseline (the reference's own evolveEulerian when oracle/_ref is built, else
 the oracle restatement) timed on a bounded sample on this host.
 """
@@ -245,6 +246,37 @@ def run_ensemble(gl, steps, warmup):
     return out
 
 
+def run_c2(steps):
+    """The 128^3 FP64 configuration run end to end: Mt 0.3, Re_lambda 50, k0 4,
+    seed 0, 2^18 seeded tracers, `steps` coupled steps (device CFL dt, Eulerian
+    step, tracer sample and Heun step each step), tiles autotuned; timed from
+    the first step to the last, state and tracers resident."""
+    from dns_gkm_lpt_amd import _lib as gl
+    from dns_gkm_lpt_amd import ic as icm
+
+    nc, ntr = 128, 1 << 18
+    ic = icm.make_ic(nc, k0=4.0, Mt=0.3, Re=50.0, seed=0)
+    P = ic["params"]
+    W = icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"], gam=P["gam"])
+    x0 = np.ascontiguousarray(icm.uniform_particles(ntr, 0)[:, [1, 0, 2]])
+    with gl.Context(nc, P, cfl=0.5, precision=gl.GKM_FP64) as c:
+        c.upload(W)
+        tiles, _ = c.autotune(reps=2)
+        c.upload(W)
+        c.lpt_upload(x0)
+        c.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            c.lpt_coupled_step(c.cfl_dt())
+        c.sync()
+        el = time.perf_counter() - t
+    out = {"grid": "128^3", "precision": "fp64", "tracers": ntr, "steps": steps, "s_run": el,
+           "ms_per_coupled_step": 1e3 * el / steps, "cell_updates_per_s": nc ** 3 * steps / el,
+           "particle_updates_per_s": ntr * steps / el, "sweep_tiles": list(tiles)}
+    log(f"bench: 128^3 run {out}")
+    return out
+
+
 def run_adjoint(steps):
     """The discrete-adjoint configuration: dJ/du0 of the final kinetic energy
     J = <rho |u|^2 / 2> with respect to the initial velocity, 64^3 periodic
@@ -287,6 +319,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 64-member ensemble configuration (0: skip)")
+    ap.add_argument("--c2-steps", type=int, default=500,
+                    help="coupled steps of the 128^3 FP64 configuration with 2^18 tracers (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the 64^3 discrete-adjoint gradient configuration (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps timed per kernel with HIP events")
@@ -375,6 +409,8 @@ def main():
         out["particles"] = particles
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(gl, args.ensemble_steps, args.warmup)
+    if args.c2_steps > 0:
+        out["run_128"] = run_c2(args.c2_steps)
     if args.adjoint_steps > 0:
         out["adjoint"] = run_adjoint(args.adjoint_steps)
     if not args.no_cpu_baseline:

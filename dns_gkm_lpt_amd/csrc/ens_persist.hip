@@ -102,8 +102,10 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
 {
     __shared__ double lds[kSweepLdsMax];
     const int G = gridDim.x;
-    int lb = blockIdx.x;  // XCD-grouped order: workgroups are dealt round-robin over the 8 XCDs
-    if ((G & 7) == 0) lb = (lb & 7) * (G >> 3) + (lb >> 3);
+    // launch order: consecutive workgroups take consecutive slices of each
+    // phase's work and are dealt round-robin over the 8 XCDs, so every XCD gets
+    // the same mix of members (their slices differ in cost)
+    const int lb = blockIdx.x;
     auto range = [&](int n, int& lo, int& hi) {
         lo = (int)((long)n * lb / G);
         hi = (int)((long)n * (lb + 1) / G);

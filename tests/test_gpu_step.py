@@ -145,7 +145,7 @@ def test_nonphysical_state_reported():
     assert ei.value.code == 4
 
 
-@pytest.mark.parametrize("nc", [32, 48])
+@pytest.mark.parametrize("nc", [32, 48, 64])
 def test_tile_variants_agree(nc):
     """Every sweep tile shape evaluates the same per-face formulas; each
     instantiation is compiled on its own (the compiler may fuse multiply-adds

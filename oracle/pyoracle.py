@@ -5,6 +5,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, as the checker.  The product package never
 imports this module.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -51,38 +52,55 @@ _lib = None
 _ref = None
 
 
+def _configure(L):
+    G = C.POINTER(Gas)
+    L.orc_weno_batch.argtypes = [C.c_long, _dp, G, _dp]
+    L.orc_flux_batch.argtypes = [C.c_long, _dp, _dp, C.c_double, G, _dp]
+    L.orc_tau.argtypes = [_dp, G]
+    L.orc_tau.restype = C.c_double
+    L.orc_halo_fill.argtypes = [C.c_int, _dp]
+    L.orc_fluxes.argtypes = [C.c_int, _dp, C.c_double, G, C.c_int, _dp, _dp, _dp]
+    L.orc_update.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double]
+    L.orc_eulerian_step.argtypes = [C.c_int, _dp, C.c_double, G, C.c_int]
+    L.orc_cfl_dt.argtypes = [C.c_int, _dp, C.c_double, G]
+    L.orc_cfl_dt.restype = C.c_double
+    L.orc_spline_coefs.argtypes = [C.c_int, _dp, _dp]
+    L.orc_spline_eval.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_velocity_coefs.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_particle_sample.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_long, _dp, _dp,
+                                      _dp, _dp, _dp]
+    L.orc_particle_heun.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_long, _dp, _dp,
+                                    C.c_double]
+    L.orc_lap_field.argtypes = [C.c_int, _dp, C.c_double, _dp]
+    L.orc_particle_props.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_long, _dp, _dp]
+    L.orc_field_stats.argtypes = [C.c_int, _dp, _dp, _dp, _dp, G, C.c_double, C.c_double,
+                                  C.c_double, C.POINTER(Stats)]
+    L.orc_particle_stats.argtypes = [C.c_long, C.c_long, _dp, _dp, _dp, _dp, C.POINTER(PStats)]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        G = C.POINTER(Gas)
-        L.orc_weno_batch.argtypes = [C.c_long, _dp, G, _dp]
-        L.orc_flux_batch.argtypes = [C.c_long, _dp, _dp, C.c_double, G, _dp]
-        L.orc_tau.argtypes = [_dp, G]
-        L.orc_tau.restype = C.c_double
-        L.orc_halo_fill.argtypes = [C.c_int, _dp]
-        L.orc_fluxes.argtypes = [C.c_int, _dp, C.c_double, G, C.c_int, _dp, _dp, _dp]
-        L.orc_update.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double]
-        L.orc_eulerian_step.argtypes = [C.c_int, _dp, C.c_double, G, C.c_int]
-        L.orc_cfl_dt.argtypes = [C.c_int, _dp, C.c_double, G]
-        L.orc_cfl_dt.restype = C.c_double
-        L.orc_spline_coefs.argtypes = [C.c_int, _dp, _dp]
-        L.orc_spline_eval.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
-                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.orc_velocity_coefs.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
-        L.orc_particle_sample.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_long, _dp, _dp,
-                                          _dp, _dp, _dp]
-        L.orc_particle_heun.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_long, _dp, _dp,
-                                        C.c_double]
-        L.orc_lap_field.argtypes = [C.c_int, _dp, C.c_double, _dp]
-        L.orc_particle_props.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_long, _dp, _dp]
-        L.orc_field_stats.argtypes = [C.c_int, _dp, _dp, _dp, _dp, G, C.c_double, C.c_double,
-                                      C.c_double, C.POINTER(Stats)]
-        L.orc_particle_stats.argtypes = [C.c_long, C.c_long, _dp, _dp, _dp, _dp, C.POINTER(PStats)]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB))
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every helper below through the oracle library at `path` (a
+    variant built from a modified copy of gkm_oracle.c, for the tests that
+    check a pin fails on a planted mistake)."""
+    global _lib
+    saved = _lib
+    _lib = _configure(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def ref_available():

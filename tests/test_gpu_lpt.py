@@ -107,3 +107,43 @@ def test_particle_properties_match_oracle():
     worst = int(np.argmax(err))
     print("props worst column", gl.PROP_NAMES[worst], err[worst], flush=True)
     assert np.all(err < 1e-10), dict(zip(gl.PROP_NAMES, err))
+
+
+def test_field_stats_closed_forms():
+    """The GPU statistics on the closed-form fields that pin the oracle's
+    dissipation and rms normalisations (test_oracle_stats_pins.py): Taylor-Green
+    at T(z) = 380 (1 + 0.3 cos 2z) K with Sutherland viscosity, and a fluid at
+    rest with p = p0 (1 + 0.07 sin x), rho = 1.3 (1 + 0.12 sin y)."""
+    R = 287.0
+    nc = 64
+    dx = 2 * np.pi / nc
+    c = (np.arange(nc) + 0.5) * dx
+    y, x, z = np.meshgrid(c, c, c, indexing="ij")
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    T = 380.0 * (1 + 0.3 * np.cos(2 * z))
+    W = icm.prim_to_cons(u, v, np.zeros_like(u), np.full_like(u, 0.9), 0.9 * R * T)
+    P = {"gam": 1.4, "rho0": 1.0, "Mt": 0.3, "Re": 30.0, "mu0": 2.0e-3, "p0": 86100.0, "t0": 1.0}
+    with gl.Context(nc, P) as cx:
+        cx.upload(W)
+        s = cx.stats()
+    Tz = 380.0 * (1 + 0.3 * np.cos(2 * c))
+    mu = 2.0e-3 * (Tz / 300.0) ** 1.5 * (300.0 + 110.4) / (Tz + 110.4)
+    expect = np.mean(mu * (np.cos(c) ** 2 + 0.5 * np.sin(c) ** 2))
+    assert s.diss == pytest.approx(expect, rel=2e-3)
+
+    nc = 32
+    c = (np.arange(nc) + 0.5) * 2 * np.pi / nc
+    y, x, _ = np.meshgrid(c, c, c, indexing="ij")
+    eps, dlt, p0, r0, Mt = 0.07, 0.12, 86100.0, 1.3, 0.45
+    zero = np.zeros_like(x)
+    W = icm.prim_to_cons(zero, zero, zero, r0 * (1 + dlt * np.sin(y)), p0 * (1 + eps * np.sin(x)))
+    P = {"gam": 1.4, "rho0": r0, "Mt": Mt, "Re": 30.0, "mu0": 1e-3, "p0": p0, "t0": 1.0}
+    with gl.Context(nc, P) as cx:
+        cx.upload(W)
+        s = cx.stats()
+    a1, a2 = (1 - dlt ** 2) ** -0.5, (1 - dlt ** 2) ** -1.5
+    T0 = p0 / (R * r0)
+    assert s.prms == pytest.approx(eps / np.sqrt(2) / (1.4 * Mt * Mt), rel=1e-11)
+    assert s.trms == pytest.approx(T0 * np.sqrt((1 + eps ** 2 / 2) * a2 - a1 ** 2) / (0.4 * T0 * Mt * Mt), rel=1e-10)
+    assert s.vrms == pytest.approx(np.sqrt(a2 - a1 ** 2) / r0 / (Mt * Mt / r0), rel=1e-10)

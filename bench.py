@@ -9,8 +9,9 @@ with the update) of the periodic cube, state resident in HBM.  The workload is
 the 512^3 mixed-precision case (Mt 0.5, Re_lambda 100, k0 4, seed 0) with 2^22
 tracers; the FP64 step of the same state is timed beside it ("FP64 vs mixed").
 Prints one JSON line: cell-updates/s of the mixed step (`value`), the FP64
-figures, the HBM-roofline fraction of the dominant kernel (HIP events on the
-solver's stream), its VALU rate against the reference's operation count,
+figures, the roofline of the dominant kernel (HIP events on the solver's
+stream): its VALU rate against the FP64 peak with the reference's operation
+count, and beside it its HBM fraction,
 particle-updates/s, the 64-member ensemble (per-phase graph, eager and
 persistent-kernel steps), the 128^3 FP64 run with 2^18 tracers (500 coupled
 steps), the 64^3 discrete-adjoint gradient, and the CPU
@@ -81,10 +82,58 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline():
-    """Time the CPU Eulerian step on a bounded sample of the same workload
-    (one step of a 128^3 cube of the same initial-condition recipe, ~10-20 s on
-    one core)."""
+def cpu_info():
+    """Host CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def _cpu_step(W, nc, P, dt, use_ref):
+    """One CPU Eulerian step of the padded state W; returns the wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    if use_ref:
+        R = po.ref()
+        R.ref_set_params(nc, P["gam"], P["mu0"], P["p0"], P["rho0"], dt)
+        Wc = np.ascontiguousarray(W.copy())
+        t = time.perf_counter()
+        R.ref_eulerian(Wc)
+        return time.perf_counter() - t
+    gas = po.make_gas(nc, gam=P["gam"], mu0=P["mu0"], p0=P["p0"], den0=P["rho0"])
+    t = time.perf_counter()
+    po.eulerian_step(W, dt, gas, nthreads=1)
+    return time.perf_counter() - t
+
+
+def _cpu_worker(args):
+    W, nc, P, dt, use_ref, q = args
+    q.put(_cpu_step(W, nc, P, dt, use_ref))
+
+
+def cpu_baseline(max_workers=16):
+    """Time the CPU Eulerian step on a bounded sample of the same workload: one
+    step of a 128^3 cube of the same initial-condition recipe (Mt 0.5, Re_lambda
+    100, k0 4, seed 0), with the reference's own evolveEulerian when
+    oracle/_ref is built (else the oracle restatement), (1) on one core and (2)
+    as one independent single-threaded step per usable core (at most
+    max_workers) run at once -- what the reference's MPI path gets from the
+    same cores when each rank owns a block (its halo exchange not counted).
+    Runs in forked worker processes before this process touches the GPU."""
+    import multiprocessing as mp
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from dns_gkm_lpt_amd import ic as icm
@@ -95,23 +144,35 @@ def cpu_baseline():
     W = icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"])
     gas = po.make_gas(nc, gam=P["gam"], mu0=P["mu0"], p0=P["p0"], den0=P["rho0"])
     dt = po.cfl_dt(W, 0.5, gas)
-    R = po.ref()
-    if R is not None:
-        R.ref_set_params(nc, P["gam"], P["mu0"], P["p0"], P["rho0"], dt)
-        Wc = np.ascontiguousarray(W.copy())
-        t = time.perf_counter()
-        R.ref_eulerian(Wc)
-        el = time.perf_counter() - t
-        kind = "reference"
-        what = "reference evolveEulerian (GKMflux.cpp + update.cpp CPU sweeps, 1 thread)"
-    else:
-        t = time.perf_counter()
-        po.eulerian_step(W, dt, gas, nthreads=1)
-        el = time.perf_counter() - t
-        kind = "port"
-        what = "oracle restatement of evolveEulerian (1 thread)"
-    return {"value": nc ** 3 / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"1 step of {nc}^3 (Mt 0.5, Re_lambda 100, k0 4, seed 0), {what}, {el:.2f} s"}
+    use_ref = po.ref_available()
+    info = cpu_info()
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    # (1) one core
+    w = ctx.Process(target=_cpu_worker, args=((W, nc, P, dt, use_ref, q),))
+    w.start()
+    el1 = q.get()
+    w.join()
+    # (2) every usable core at once
+    nw = max(1, min(max_workers, info["usable"]))
+    t = time.perf_counter()
+    ws = [ctx.Process(target=_cpu_worker, args=((W, nc, P, dt, use_ref, q),)) for _ in range(nw)]
+    for w in ws:
+        w.start()
+    els = [q.get() for _ in ws]
+    for w in ws:
+        w.join()
+    wall = time.perf_counter() - t
+    kind = "reference" if use_ref else "port"
+    what = ("the reference's evolveEulerian (GKMflux.cpp + update.cpp CPU sweeps, oracle/_ref)" if use_ref
+            else "the oracle restatement of evolveEulerian")
+    out = {"value": nw * nc ** 3 / wall, "unit": "cell-updates/s", "cores": nw, "kind": kind,
+           "sample": f"{nw} concurrent single-threaded steps of {nc}^3 (Mt 0.5, Re_lambda 100, k0 4, seed 0), "
+                     f"one per core, {what}; {wall:.2f} s wall (per-step {min(els):.2f}-{max(els):.2f} s)",
+           "cpu_model": info["model"], "nproc": info["nproc"], "usable_cores": info["usable"],
+           "one_core": {"value": nc ** 3 / el1, "s_step": el1}}
+    log(f"bench: CPU baseline {out}")
+    return out
 
 
 def copy_calibration(mib):
@@ -206,13 +267,8 @@ def run_ensemble(gl, steps, warmup):
     persistent-kernel launch per step (also graph-replayed)."""
     from dns_gkm_lpt_amd import ic as icm
 
-    rng = np.random.default_rng(4)
-    ncs = [32] * 24 + [48] * 24 + [64] * 16
-    cases = []
-    for m, nc in enumerate(ncs):
-        k0, Mt, Re = rng.uniform(3, 6), rng.uniform(0.1, 0.6), rng.uniform(20, 60)
-        ic = icm.make_ic(nc, k0=k0, Mt=Mt, Re=Re, seed=100 + m)
-        cases.append((nc, ic["params"], icm.prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"])))
+    cases = icm.ensemble_members()
+    ncs = [c[0] for c in cases]
     cells = sum(nc ** 3 for nc in ncs)
     ntr = 8192
     out = {"members": len(ncs), "grids": "24 x 32^3 + 24 x 48^3 + 16 x 64^3", "precision": "mixed",
@@ -220,7 +276,7 @@ def run_ensemble(gl, steps, warmup):
     with gl.Ensemble(ncs, [c[1] for c in cases], cfl=0.5, precision=gl.GKM_MIXED) as e:
         for m, c in enumerate(cases):
             e.members[m].upload(c[2])
-            e.members[m].lpt_upload(icm.uniform_particles(ntr, seed=100 + m))
+            e.members[m].lpt_upload(icm.uniform_particles(ntr, seed=c[3]))
         tiles, _ = e.autotune(reps=2)
         e.run(warmup, graph=True)
         for mode, graph, persistent in (("graph", True, False), ("eager", False, False),
@@ -336,6 +392,11 @@ def main():
     if args.profile_steps < 1 or args.steps < 1:
         ap.error("--steps and --profile-steps must be >= 1")
 
+    cpu = None
+    if not args.no_cpu_baseline:  # first: its worker processes fork before this process touches the GPU
+        log("bench: timing the CPU baseline")
+        cpu = cpu_baseline()
+
     from dns_gkm_lpt_amd import _lib as gl
     from dns_gkm_lpt_amd import ic as icm
 
@@ -364,6 +425,7 @@ def main():
     dom = max(range(3), key=lambda i: ms[i] / max(cnt[i], 1))
     dom_ms = ms[dom] / cnt[dom]
     achieved = SWEEP_BYTES[dom] * nc ** 3 / (dom_ms * 1e-3) / 1e9
+    hbm80 = STEP_BYTES * nc ** 3 / (dom_ms * 1e-3) / 1e9
     alg_tops = ALG_OPS_PER_FACE * nc ** 3 / (dom_ms * 1e-3) / 1e12
     measured = MEASURED_TRAFFIC_GIB.get((precs[-1], dom, head["sweep_tiles"][dom], nc))
     traffic = None if measured is None else (measured[0] + measured[1]) * 2.0 ** 30
@@ -384,20 +446,28 @@ def main():
         "config": {"workload": f"GKM-WENO Eulerian step (CFL 0.5 dt on device + fused x/y/z sweeps + update), "
                                f"{nc}^3 periodic cube, Mt 0.5, Re_lambda 100, k0 4, seed 0",
                    "nc": nc, "precision": precs[-1], "sweep_tiles": head["sweep_tiles"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        # The sweep is VALU-bound (about 340 reference operations per algorithmic
+        # byte against a ridge near 10): the roofline is the FP64 VALU peak with
+        # the reference's operation count; the HBM figures stand beside it.
+        "roofline": {"bound": "valu", "achieved": alg_tops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": alg_tops / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": KERNEL_NAMES[dom], "avg_ms": dom_ms, "alg_ops_per_face": ALG_OPS_PER_FACE,
+                     "note": "achieved = reference operations per face (oracle/count_ops.cpp, one WENO per "
+                             "interface) x nc^3 faces / average launch time (HIP events on the solver's "
+                             "stream); peak = MI355X FP64 vector peak (non-packed FP32 issues at the same "
+                             "rate)",
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
                                      "calibrated on a 1 GiB copy; profiles/sweep_traffic_512.md)"
                                      if traffic else "no counter measurement for this configuration",
-                     "kernel": KERNEL_NAMES[dom], "avg_ms": dom_ms,
-                     "algorithmic_bytes_per_cell": SWEEP_BYTES[dom],
-                     "valu": {"alg_ops_per_face": ALG_OPS_PER_FACE, "achieved_tops": alg_tops,
-                              "fp64_peak_tflops": FP64_PEAK_TFLOPS, "frac_of_fp64_peak": alg_tops / FP64_PEAK_TFLOPS,
-                              "note": "reference operation count (one WENO per interface) over the kernel time",
-                              "valu_busy_pct_rocprof": MEASURED_VALU_BUSY.get(
-                                  (precs[-1], dom, head["sweep_tiles"][dom], nc)),
-                              "valu_busy_note": "rocprofv3 VALUBusy of this kernel configuration "
-                                                "(profiles/sweep_valu_512.md)"}},
+                     "hbm": {"achieved_gbs": hbm80, "frac": hbm80 / HBM_PEAK_GBS, "bytes_per_cell": STEP_BYTES,
+                             "launch_alg_bytes_per_cell": SWEEP_BYTES[dom],
+                             "launch_alg_gbs": achieved, "launch_alg_frac": achieved / HBM_PEAK_GBS,
+                             "traffic_over_launch_alg": None if traffic is None else
+                             traffic / (SWEEP_BYTES[dom] * nc ** 3),
+                             "peak_gbs": HBM_PEAK_GBS},
+                     "valu_busy_pct_rocprof": MEASURED_VALU_BUSY.get((precs[-1], dom, head["sweep_tiles"][dom], nc)),
+                     "valu_busy_note": "rocprofv3 VALUBusy of this kernel configuration "
+                                       "(profiles/sweep_valu_512.md)"},
         "step_hbm_fraction": STEP_BYTES * head["value"] / 1e9 / HBM_PEAK_GBS,
         "kernels": head["kernels"],
     }
@@ -413,9 +483,8 @@ def main():
         out["run_128"] = run_c2(args.c2_steps)
     if args.adjoint_steps > 0:
         out["adjoint"] = run_adjoint(args.adjoint_steps)
-    if not args.no_cpu_baseline:
-        log("bench: timing the CPU baseline")
-        out["cpu_baseline"] = cpu_baseline()
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -25,7 +25,7 @@ EXPORTS = [
     "gkm_ensemble_set_tiles", "gkm_ensemble_autotune", "gkm_ensemble_run", "gkm_ensemble_dt",
     "gkm_ensemble_set_persistent", "gkm_ensemble_persistent_profile",
     "gkm_stats_compute", "gkm_timing_enable", "gkm_timing_read", "gkm_flux_batch", "gkm_weno_batch",
-    "lpt_upload", "lpt_download", "lpt_sample", "lpt_stats", "lpt_properties", "lpt_coupled_step", "lpt_heun",
+    "lpt_upload", "lpt_set_reference_drop", "lpt_count", "lpt_download", "lpt_sample", "lpt_stats", "lpt_properties", "lpt_coupled_step", "lpt_heun",
 ]
 
 
@@ -112,6 +112,8 @@ def lib():
         L.gkm_flux_batch.argtypes = [P, C.c_long, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp]
         L.gkm_weno_batch.argtypes = [P, C.c_long, _dp, C.c_int, _dp]
         L.lpt_upload.argtypes = [P, C.c_long, _dp]
+        L.lpt_set_reference_drop.argtypes = [P, C.c_int]
+        L.lpt_count.argtypes = [P, C.POINTER(C.c_long)]
         L.lpt_download.argtypes = [P, C.c_void_p, C.c_void_p]
         L.lpt_sample.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.lpt_stats.argtypes = [P, C.POINTER(PStats)]
@@ -280,6 +282,16 @@ class Context:
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         self.np = pos.shape[0]
         self._chk(lib().lpt_upload(self._h, self.np, pos))
+
+    def lpt_set_reference_drop(self, on=True):
+        """Drop tracers at block id != 0 (a coordinate of exactly 2 pi) as the
+        reference's single-rank run does; call before lpt_upload."""
+        self._chk(lib().lpt_set_reference_drop(self._h, int(on)))
+
+    def lpt_count(self):
+        n = C.c_long()
+        self._chk(lib().lpt_count(self._h, C.byref(n)))
+        return n.value
 
     def lpt_download(self):
         pos = np.empty((self.np, 3))

@@ -17,10 +17,13 @@
 //
 // Grid barrier: every workgroup adds one to a monotonic counter per barrier
 // and waits until it reaches base + k G (base: the counter when the launch
-// started, republished by workgroup 0 after the last barrier).  Each wait is
-// bounded by a wall-clock limit: on expiry the workgroup sets the timeout flag
-// and carries on, so every wave reaches the end of the kernel even if the
-// workgroups were not all resident; the host then reports an error.
+// started, republished by workgroup 0 after the last barrier).  Eager steps
+// are cooperative launches, which fail up front when the grid cannot be
+// resident at once; graph-captured steps are ordinary launches of the same
+// occupancy-sized grid.  Each wait is bounded by a wall-clock limit: on expiry
+// the workgroup sets the timeout flag and carries on, every other workgroup
+// stops waiting as soon as it sees the flag, later launches of the run return
+// at entry, and the host reports the error and resets the barrier.
 #include "ens_persist.hpp"
 #include "gkm_sweep.hpp"
 #include "lpt_device.hpp"
@@ -44,18 +47,22 @@ __device__ __forceinline__ int entry_of(const int* __restrict__ first, int n, in
 // workgroup then arrives with an agent-scope release (the XCD's L2 is written
 // back before the counter moves), polls the counter with relaxed loads, and
 // after the wait invalidates this CU's L1 and the L2 (agent-scope acquire)
-// before the workgroup continues.
-__device__ __forceinline__ void grid_barrier(const PersistStep& a, unsigned long long target)
+// before the workgroup continues.  Once any workgroup has timed out (the flag
+// is set) the others stop waiting: they neither arrive nor wait at this or any
+// later barrier, so a failed launch ends after one wait limit.
+// Returns true (in every thread of the workgroup) when the launch has failed.
+__device__ __forceinline__ bool grid_barrier(const PersistStep& a, unsigned long long target, int* failed)
 {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && __hip_atomic_load(a.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // L2 write-back
         __builtin_amdgcn_s_waitcnt(0);                      // ... completed before the arrival is seen
         __hip_atomic_fetch_add(a.count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long t0 = wall_clock64();
         while (__hip_atomic_load(a.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(2);
+            if (__hip_atomic_load(a.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
             if (wall_clock64() - t0 > a.timeout_ticks) {
                 __hip_atomic_store(a.timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -63,7 +70,9 @@ __device__ __forceinline__ void grid_barrier(const PersistStep& a, unsigned long
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    if (threadIdx.x == 0) *failed = __hip_atomic_load(a.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
+    return *failed != 0;
 }
 
 template <int DIR, bool MIX, int V>
@@ -101,6 +110,12 @@ template <bool MIX>
 __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
 {
     __shared__ double lds[kSweepLdsMax];
+    __shared__ int failed;
+    // a launch after a timed-out one (same run, flag not yet cleared by the
+    // host) does nothing; thread 0 reads the flag for the whole workgroup
+    if (threadIdx.x == 0) failed = __hip_atomic_load(a.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (failed) return;
     const int G = gridDim.x;
     // launch order: consecutive workgroups take consecutive slices of each
     // phase's work and are dealt round-robin over the 8 XCDs, so every XCD gets
@@ -113,9 +128,11 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
     const unsigned long long base = __hip_atomic_load(a.base, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     int nbar = 0;
     if (a.trace && blockIdx.x == 0 && threadIdx.x == 0) a.trace[0] = wall_clock64();
+    // true: a barrier of this launch timed out, and the workgroup ends here
     auto sync = [&] {
-        grid_barrier(a, base + (unsigned long long)(++nbar) * G);
+        if (grid_barrier(a, base + (unsigned long long)(++nbar) * G, &failed)) return true;
         if (a.trace && blockIdx.x == 0 && threadIdx.x == 0) a.trace[nbar] = wall_clock64();
+        return false;
     };
     const int tid = threadIdx.x;
     int lo, hi;
@@ -134,26 +151,26 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
         const long p = (long)(i - a.pfirst[e]) * 256 + tid;
         if (p < P.np) lpt::sample_one(p, P.pos, P.c[0], P.c[1], P.c[2], P.g, P.U, P.G);
     }
-    sync();
+    if (sync()) return;
     // 2: dt of every member
     range(a.nmem, lo, hi);
     for (int m = lo; m < hi; m++) {
         const CflArgs c = a.cfl[m];
         cfl_reduce(a.partial + (long)m * kEnsCflBlocks, kEnsCflBlocks, c.cfl, c.dx, c.dt, c.vmax, c.bad);
     }
-    sync();
+    if (sync()) return;
     // 3-5: sweeps
     range(a.nsweep[0], lo, hi);
     sweep_items<0, MIX>(a, lo, hi, lds);
-    sync();
+    if (sync()) return;
     range(a.nsweep[1], lo, hi);
     sweep_items<1, MIX>(a, lo, hi, lds);
-    sync();
+    if (sync()) return;
     range(a.nsweep[2], lo, hi);
     sweep_items<2, MIX>(a, lo, hi, lds);
     if (a.npart) {
         // 6: z pass of the splines of u, v, w of W^{n+1}
-        sync();
+        if (sync()) return;
         range(a.zblk, lo, hi);
         for (int i = lo; i < hi; i++) {
             switch (a.zchunks) {
@@ -167,7 +184,7 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
         }
         // 7, 8: x and y passes; an item is four 64-line groups, one per wave
         for (int axis = 1; axis <= 2; axis++) {
-            sync();
+            if (sync()) return;
             const int nsub = a.lblk[axis - 1];
             const int* first = a.lfirst + (axis - 1) * (3 * a.npart + 1);
             range((nsub + 3) / 4, lo, hi);
@@ -180,7 +197,7 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
             }
         }
         // 9: Heun
-        sync();
+        if (sync()) return;
         range(a.pblk, lo, hi);
         for (int i = lo; i < hi; i++) {
             const int e = lpt::ens_entry(a.pfirst, a.npart, i);
@@ -190,7 +207,7 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
         }
     }
     if (a.trace) {  // the end of the last phase
-        sync();
+        if (sync()) return;
     }
     // every workgroup reached the last barrier, so every one has read base:
     // publish the base of the next launch (the counter ends at base + nbar G)
@@ -212,9 +229,17 @@ int persist_grid(bool mixed, int device)
     return g;
 }
 
-hipError_t launch_persist_step(const PersistStep& a, bool mixed, int grid, hipStream_t s)
+hipError_t launch_persist_step(const PersistStep& a, bool mixed, int grid, bool cooperative, hipStream_t s)
 {
     if (grid < 1 || a.nmem < 1) return hipErrorInvalidValue;
+    const void* k = mixed ? reinterpret_cast<const void*>(&ens_step_kernel<true>)
+                          : reinterpret_cast<const void*>(&ens_step_kernel<false>);
+    if (cooperative) {
+        // fails up front when the grid cannot be resident at once
+        PersistStep arg = a;
+        void* params[] = {&arg};
+        return hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), params, 0, s);
+    }
     if (mixed)
         hipLaunchKernelGGL(ens_step_kernel<true>, dim3(grid), dim3(256), 0, s, a);
     else

@@ -42,6 +42,7 @@ struct PersistStep {
 
 // workgroups of the persistent kernel that are resident at once (0: unknown)
 int persist_grid(bool mixed, int device);
-hipError_t launch_persist_step(const PersistStep& a, bool mixed, int grid, hipStream_t s);
+// cooperative: launch with hipLaunchCooperativeKernel (not inside a graph capture)
+hipError_t launch_persist_step(const PersistStep& a, bool mixed, int grid, bool cooperative, hipStream_t s);
 
 }  // namespace gkm

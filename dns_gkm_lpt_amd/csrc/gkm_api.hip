@@ -295,6 +295,11 @@ int gkm_run(gkm_ctx* c, int nsteps, double* dt_hist)
         rc = eulerian_step(c);
         if (rc) return rc;
     }
+    // the flag so far describes W^0 .. W^{n-1}; check the state the last step produced
+    if (nsteps > 0) {
+        int rc = cfl_on_device(c);
+        if (rc) return rc;
+    }
     int flag = 0;
     HIPCHK(c, hipMemcpyAsync(&flag, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -569,6 +574,7 @@ lpt::Geom geom_of(const gkm_ctx* c)
     const double e = s + (g.nt - 1) * c->dx;
     g.start = s;
     g.dinv = 1.0 / ((e - s) / (double)(g.nt - 1));
+    g.drop = c->lpt.drop;
     return g;
 }
 
@@ -700,6 +706,17 @@ int lpt_upload(gkm_ctx* c, long np, const double* pos)
         return fail(c, GKM_ENOMEM, "hipMalloc particles");
     std::vector<int> tags(np);
     for (long p = 0; p < np; p++) tags[p] = (int)p;
+    std::vector<double> kept;
+    if (L.drop) {  // locateParticles keeps a tracer only in the block that owns it (update.cpp:1304-1316)
+        kept.assign(pos, pos + 3 * np);
+        for (long p = 0; p < np; p++) {
+            bool out = false;
+            for (int a = 0; a < 3; a++) out |= (int)(kept[3 * p + a] / twopi) != 0;
+            if (out)
+                for (int a = 0; a < 3; a++) kept[3 * p + a] = std::nan("");
+        }
+        pos = kept.data();
+    }
     HIPCHK(c, hipMemcpyAsync(L.pos, pos, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(L.tag, tags.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, lpt::sort_by_cell(L, geom_of(c), c->dx, c->stream));
@@ -714,6 +731,29 @@ static int download_tag_order(gkm_ctx* c, const double* src, int width, double* 
     HIPCHK(c, lpt::scatter_to_tags(L.np, L.tag, src, width, L.pos2, c->stream));
     HIPCHK(c, hipMemcpyAsync(host, L.pos2, sizeof(double) * width * L.np, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
+int lpt_set_reference_drop(gkm_ctx* c, int on)
+{
+    if (!c) return GKM_EINVAL;
+    if (c->lpt.np > 0) return fail(c, GKM_ENOTREADY, "set the tracer ownership mode before lpt_upload");
+    c->lpt.drop = on != 0;
+    return GKM_OK;
+}
+
+int lpt_count(gkm_ctx* c, long* alive)
+{
+    if (!c || !alive) return fail(c, GKM_EINVAL, "null argument");
+    const lpt::State& L = c->lpt;
+    *alive = L.np;
+    if (L.np == 0 || !L.drop) return GKM_OK;
+    std::vector<double> pos((size_t)3 * L.np);
+    HIPCHK(c, hipMemcpyAsync(pos.data(), L.pos, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    long n = 0;
+    for (long p = 0; p < L.np; p++) n += !std::isnan(pos[(size_t)3 * p]);
+    *alive = n;
     return GKM_OK;
 }
 
@@ -1041,10 +1081,13 @@ gkm::PersistStep persist_args(const gkm_ensemble* e, int p)
     return a;
 }
 
-hipError_t ens_step(gkm_ensemble* e, int p)
+// capturing: the launches go into a HIP graph (the persistent step is then an
+// ordinary launch; eager persistent steps are cooperative launches)
+hipError_t ens_step(gkm_ensemble* e, int p, bool capturing)
 {
     const int n = e->n, np = e->npart;
-    if (e->persistent) return gkm::launch_persist_step(persist_args(e, p), e->mixed, e->pgrid, e->stream);
+    if (e->persistent)
+        return gkm::launch_persist_step(persist_args(e, p), e->mixed, e->pgrid, !capturing, e->stream);
     hipError_t r = hipSuccess;
     if (np) r = lpt::launch_ens_sample(e->dpart + (size_t)(2 * p) * np, e->dpfirst, np, e->pblk, e->stream);
     if (r == hipSuccess) r = gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream);
@@ -1314,7 +1357,7 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
             if (e->graph[q]) continue;
             hipGraph_t g = nullptr;
             EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t r = ens_step(e, q);
+            const hipError_t r = ens_step(e, q, true);
             const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
             if (r != hipSuccess || r2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1332,7 +1375,7 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
         if (use_graph)
             EHIPCHK(e, hipGraphLaunch(e->graph[p], e->stream));
         else
-            EHIPCHK(e, ens_step(e, p));
+            EHIPCHK(e, ens_step(e, p, false));
         p = 1 - p;
     }
     EHIPCHK(e, hipStreamSynchronize(e->stream));
@@ -1340,8 +1383,9 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
         int timed_out = 0;
         EHIPCHK(e, hipMemcpy(&timed_out, e->dbar + 2, sizeof(int), hipMemcpyDeviceToHost));
         if (timed_out) {
-            const int zero = 0;
-            EHIPCHK(e, hipMemcpy(e->dbar + 2, &zero, sizeof(int), hipMemcpyHostToDevice));
+            // the counter and base of the failed launches are stale: start the
+            // barrier afresh (graph replays read the same device words)
+            EHIPCHK(e, hipMemset(e->dbar, 0, 3 * sizeof(unsigned long long)));
             return efail(e, GKM_EHIP, "persistent step: a grid barrier timed out (workgroups not all resident)");
         }
     }
@@ -1399,7 +1443,7 @@ int gkm_ensemble_persistent_profile(gkm_ensemble* e, double ms[9], int* nphases)
     if (hipMalloc(&dtr, 10 * sizeof(unsigned long long)) != hipSuccess) return efail(e, GKM_ENOMEM, "hipMalloc trace");
     gkm::PersistStep a = persist_args(e, p);
     a.trace = dtr;
-    hipError_t r = gkm::launch_persist_step(a, e->mixed, e->pgrid, e->stream);
+    hipError_t r = gkm::launch_persist_step(a, e->mixed, e->pgrid, true, e->stream);
     unsigned long long t[10] = {0};
     if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
     if (r == hipSuccess) r = hipMemcpy(t, dtr, sizeof(t), hipMemcpyDeviceToHost);

@@ -268,6 +268,14 @@ __device__ __forceinline__ void sample_one(long p, const double* __restrict__ po
                                            const double* __restrict__ cv, const double* __restrict__ cw, const Geom& g,
                                            double* __restrict__ U, double* __restrict__ G)
 {
+    if (g.drop && isnan(pos[3 * p])) {  // a dropped tracer contributes nothing
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            if (U) U[3 * p + f] = 0.0;
+            if (G) G[9 * p + 3 * f + 0] = G[9 * p + 3 * f + 1] = G[9 * p + 3 * f + 2] = 0.0;
+        }
+        return;
+    }
     const Loc Ly = locate(pos[3 * p + 0], g, true);
     const Loc Lx = locate(pos[3 * p + 1], g, true);
     const Loc Lz = locate(pos[3 * p + 2], g, true);
@@ -308,6 +316,7 @@ __device__ __forceinline__ void heun_one(long p, double* __restrict__ pos, const
 {
     const double u = U[3 * p + 0], v = U[3 * p + 1], w = U[3 * p + 2];
     double y = pos[3 * p + 0], x = pos[3 * p + 1], z = pos[3 * p + 2];
+    if (g.drop && isnan(y)) return;  // dropped earlier
     const Loc Ly = locate(y + v * dt, g, false);
     const Loc Lx = locate(x + u * dt, g, false);
     const Loc Lz = locate(z + w * dt, g, false);
@@ -330,12 +339,16 @@ __device__ __forceinline__ void heun_one(long p, double* __restrict__ pos, const
     z += 0.5 * (w + U2[2]) * dt;
     const double twopi = 2.0 * 3.141592653589793238462643383279502884;
     double r[3] = {y, x, z};
+    bool out = false;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         if (r[c] > twopi) r[c] = fmod(r[c], twopi);
         if (r[c] < 0) r[c] = twopi - fmod(-r[c], twopi);
-        pos[3 * p + c] = r[c];
+        out |= (int)(r[c] / twopi) != 0;  // block id with procDim 1 (ParticleTrack.cpp:24-25)
     }
+    const double nan = __builtin_nan("");
+#pragma unroll
+    for (int c = 0; c < 3; c++) pos[3 * p + c] = (g.drop && out) ? nan : r[c];
 }
 
 __device__ __forceinline__ int ens_entry(const int* __restrict__ first, int n, int b)

@@ -205,6 +205,10 @@ __global__ __launch_bounds__(256) void props_kernel(long np, const double* __res
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
+    if (g.drop && isnan(pos[3 * p])) {  // dropped tracer: a row of NaN
+        for (int k = 0; k < kPropWidth; k++) out[(long)tag[p] * kPropWidth + k] = __builtin_nan("");
+        return;
+    }
     const Loc Ly = locate(pos[3 * p + 0], g, true);
     const Loc Lx = locate(pos[3 * p + 1], g, true);
     const Loc Lz = locate(pos[3 * p + 2], g, true);

@@ -32,6 +32,7 @@ struct State {
     size_t sort_tmp_bytes = 0;
     int sort_every = 10;       // re-sort after this many Heun steps
     int since_sort = 0;
+    int drop = 0;              // reference tracer ownership (Geom::drop)
 };
 
 void release(State& s);
@@ -41,6 +42,12 @@ void release(State& s);
 struct Geom {
     int nc, nt, N;      // N = nt + 2 coefficients per axis
     double start, dinv; // 1/delta
+    // 1: the reference's single-rank tracer ownership -- a tracer whose block
+    // id int(pos / 2 pi) is not 0 on some axis (after the wrap: a coordinate
+    // of exactly 2 pi) is dropped (its position set to NaN) as evolveParticles
+    // pops it (update.cpp:1056-1109, ParticleTrack.cpp:21-40); dropped tracers
+    // sample zero velocity and gradients and are not advanced
+    int drop;
 };
 
 // field q of the unpadded state W (0 = rho, 1..3 = velocity component,

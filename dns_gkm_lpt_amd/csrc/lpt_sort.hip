@@ -14,7 +14,8 @@ __global__ void cell_keys_kernel(long np, const double* __restrict__ pos, int nc
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        int v = (int)floor(pos[3 * p + a] * idx);
+        const double u = pos[3 * p + a] * idx;
+        int v = u == u ? (int)floor(u) : 0;  // a dropped (NaN) tracer sorts into cell 0
         c[a] = v < 0 ? 0 : (v >= nc ? nc - 1 : v);
     }
     keys[p] = ((unsigned)c[0] * nc + (unsigned)c[1]) * nc + (unsigned)c[2];

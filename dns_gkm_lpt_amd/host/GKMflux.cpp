@@ -1,17 +1,24 @@
-// GKMflux.cpp -- flux() of the reference's GKMflux.cpp:11-229 evaluated by the
-// gfx950 device flux through gkm_flux_batch; c2p of GKMflux.cpp:302-309.
-#include "GKMflux.h"
+// GKMflux.cpp -- flux() of the reference's GKMflux.cpp:11-229, evaluated on the
+// host by the same rest-frame GKS flux template the gfx950 sweeps run
+// (csrc/gkm_flux.hpp, FP64 instance), with the gas of the param.h globals
+// (K and gam as the reference reads them); c2p of GKMflux.cpp:302-309.
+//
+// gkm_flux.hpp comes first: param.h defines pi, R and eps as macros, which
+// must not reach the template's identifiers.
+#include "../csrc/gkm_flux.hpp"
 
-#include "dnsgkm_host.h"
+#include "GKMflux.h"
 
 void flux(ptype WL[5], ptype WR[5], ptype Wl[5], ptype Wr[5], ptype DWxl[5], ptype DWxr[5], ptype DWyl[5],
           ptype DWyr[5], ptype DWzl[5], ptype DWzr[5], ptype tau, ptype dt, ptype dx, ptype F[5])
 {
-    double in[50];
-    const ptype *parts[10] = {WL, WR, Wl, Wr, DWxl, DWxr, DWyl, DWyr, DWzl, DWzr};
-    for (int v = 0; v < 10; v++)
-        for (int q = 0; q < 5; q++) in[5 * v + q] = parts[v][q];
-    dnsgkm_check(gkm_flux_batch(dnsgkm_context(), 1, in, &tau, dt, dx, GKM_FP64, F), "flux");
+    const double D[6][5] = {
+        {DWxl[0], DWxl[1], DWxl[2], DWxl[3], DWxl[4]}, {DWxr[0], DWxr[1], DWxr[2], DWxr[3], DWxr[4]},
+        {DWyl[0], DWyl[1], DWyl[2], DWyl[3], DWyl[4]}, {DWyr[0], DWyr[1], DWyr[2], DWyr[3], DWyr[4]},
+        {DWzl[0], DWzl[1], DWzl[2], DWzl[3], DWzl[4]}, {DWzr[0], DWzr[1], DWzr[2], DWzr[3], DWzr[4]}};
+    // the flux reads K, gam and 1/Pr; mu0, T0 and eps enter only the collision time and WENO
+    const gkm::Gas<double> g{gam, R, 1.0 / PRN, mu0, T0, eps, K};
+    gkm::gks_flux_tf<double, double>(WL, WR, Wl, Wr, D, tau, dt, dx, g, F);
 }
 
 void c2p(ptype W[5], ptype &den, ptype U[3], ptype &P)

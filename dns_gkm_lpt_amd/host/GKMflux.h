@@ -1,6 +1,6 @@
 // GKMflux.h -- drop-in for the reference's src/GKMflux.h.  flux() evaluates the
-// gfx950 device flux (gkm_flux_batch) for one face; c2p is the conversion of
-// GKMflux.cpp:302-309.
+// FP64 GKS face flux the gfx950 sweeps use, on the host, for one face; c2p is
+// the conversion of GKMflux.cpp:302-309.
 #ifndef GKMFLUXHEADERDEF
 #define GKMFLUXHEADERDEF
 

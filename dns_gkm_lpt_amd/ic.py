@@ -110,6 +110,21 @@ def make_ic(nc, k0=4.0, Mt=0.3, Re=30.0, seed=0, kcmin=1.0, kcmax=8.0):
                             p0=float(p0), t0=float(t0)))
 
 
+def ensemble_members(seed=4):
+    """The batched-ensemble workload: 64 independent realizations, 24 at 32^3,
+    24 at 48^3 and 16 at 64^3, member m with its own seed 100 + m and k0, Mt,
+    Re_lambda drawn uniformly from [3, 6], [0.1, 0.6], [20, 60] (seeded by
+    `seed`).  Returns [(nc, params, W padded, tracer seed)]."""
+    rng = np.random.default_rng(seed)
+    ncs = [32] * 24 + [48] * 24 + [64] * 16
+    out = []
+    for m, nc in enumerate(ncs):
+        k0, Mt, Re = rng.uniform(3, 6), rng.uniform(0.1, 0.6), rng.uniform(20, 60)
+        ic = make_ic(nc, k0=k0, Mt=Mt, Re=Re, seed=100 + m)
+        out.append((nc, ic["params"], prim_to_cons(ic["u"], ic["v"], ic["w"], ic["rho"], ic["p"]), 100 + m))
+    return out
+
+
 def uniform_particles(n, seed=0):
     """Tracer positions uniform in [0, 2 pi)^3 (Part_Gen.m); columns x, y, z."""
     rng = np.random.default_rng(seed + 1_000_003)

@@ -189,6 +189,19 @@ int gkm_ensemble_dt(gkm_ensemble *e, double *dt);
 /* Upload np positions (np x 3, order y, x, z); replaces locateParticles
  * (update.cpp:1286-1322). Tags are 0..np-1 in upload order. */
 int lpt_upload(gkm_ctx *ctx, long np, const double *pos);
+/* Tracer ownership as the reference's single-rank run has it (call before
+ * lpt_upload; default 0).  on = 1: a tracer whose block id int(pos / 2 pi)
+ * (Particle::identifyBlock, ParticleTrack.cpp:21-40, procDim 1) is not 0 on
+ * some axis -- a coordinate of exactly 2 pi at upload or after the wrap of a
+ * Heun step -- is dropped, as locateParticles (update.cpp:1286-1322) and
+ * evolveParticles (update.cpp:1056-1109) drop it: its position reads NaN, it
+ * samples zero velocity and gradients (so it adds nothing to lpt_stats, which
+ * still divides by the uploaded count, as GetParticleStats divides by
+ * numparticles), lpt_properties gives it a row of NaN, and it is not
+ * advanced.  on = 0: such a tracer is kept and sampled at the box face. */
+int lpt_set_reference_drop(gkm_ctx *ctx, int on);
+/* Tracers not dropped (the reference's per-rank `count`, update.cpp:1056-1109). */
+int lpt_count(gkm_ctx *ctx, long *alive);
 /* Download positions (np x 3) and, if non-NULL, velocities (np x 3) in tag order. */
 int lpt_download(gkm_ctx *ctx, double *pos, double *vel);
 /* Sample u, v, w and their gradients (over y, x, z) at the particles from the

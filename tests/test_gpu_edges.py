@@ -125,3 +125,52 @@ def test_calls_in_the_wrong_state_fail_cleanly():
         with pytest.raises(gl.GkmError) as ei:
             e.run(1)  # member 1 has no state
         assert ei.value.code == 5
+
+
+@pytest.mark.parametrize("drop", [False, True])
+def test_reference_drop_of_tracers_at_two_pi(drop):
+    """lpt_set_reference_drop: in a uniform flow (u = -1e-15, v = 3, w = 0) a
+    tracer at x = 0 moves to x = -u dt ~ -1e-18, which the wrap maps to 2 pi
+    exactly (2 pi - 1e-18 rounds to 2 pi): block id int(2 pi / 2 pi) = 1, so
+    the reference's single-rank run pops it (update.cpp:1056-1109); a tracer
+    uploaded at x = 2 pi is not located on rank 0 (update.cpp:1304-1316).
+    Without the mode both are kept on the face x = 2 pi."""
+    nc = 16
+    z = np.zeros((nc,) * 3)
+    u, v = -1e-15, 3.0
+    W = icm.prim_to_cons(z + u, z + v, z, z + 1.1, z + 86100.0)
+    P = {"gam": 1.4, "rho0": 1.0, "Mt": 0.3, "Re": 30.0, "mu0": 1e-3, "p0": 86100.0, "t0": 1.0}
+    pos = np.array([[1.0, 0.0, 1.0],      # x -> 2 pi after one step: dropped
+                    [1.0, 1.0, 1.0],      # kept
+                    [1.0, TWO_PI, 1.0],   # at 2 pi on upload: dropped at upload
+                    [0.0, 3.0, 0.0]])     # y = 0 moves to 3 dt: kept
+    dt = 1e-3
+    with gl.Context(nc, P) as c:
+        c.upload(W)
+        if drop:
+            c.lpt_set_reference_drop(True)
+        c.lpt_upload(pos)
+        assert c.lpt_count() == (3 if drop else 4)
+        c.lpt_coupled_step(dt)
+        x, _ = c.lpt_download()
+        assert c.lpt_count() == (2 if drop else 4)
+        U, gU, gV, gW = c.lpt_sample()
+        s = c.lpt_stats()
+        props = c.lpt_properties()
+        out = c.download()
+    assert np.array_equal(out, W)  # a uniform state does not change
+    expect_y = pos[:, 0] + v * dt
+    for k in (1, 3):
+        assert x[k, 0] == pytest.approx(expect_y[k], abs=1e-14)
+        assert x[k, 1] == pytest.approx(pos[k, 1], abs=1e-14)
+        assert np.allclose(U[k], [u, v, 0.0], atol=1e-12)
+    if drop:
+        assert np.all(np.isnan(x[[0, 2]]))
+        assert np.all(U[[0, 2]] == 0.0) and np.all(gU[[0, 2]] == 0.0)
+        assert np.all(np.isnan(props[[0, 2]])) and np.all(np.isfinite(props[[1, 3]]))
+        # GetParticleStats sums over the tracers a rank holds, divides by numparticles
+        assert s.tke == pytest.approx(2 * (u * u + v * v) / 4, rel=1e-12)
+    else:
+        assert x[0, 1] == TWO_PI and x[2, 1] == TWO_PI
+        assert np.all(np.isfinite(props))
+        assert s.tke == pytest.approx(u * u + v * v, rel=1e-12)

@@ -17,7 +17,8 @@ ERRORS = {1: "GKM_EINVAL", 2: "GKM_EHIP", 3: "GKM_ENOMEM", 4: "GKM_ESTATE", 5: "
 
 # every entry point include/dnsgkm.h declares
 EXPORTS = [
-    "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_upload_state", "gkm_download_state",
+    "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_set_compensation", "gkm_get_compensation",
+    "gkm_download_carry", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
     "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune",
     "gkm_tl_run", "gkm_adjoint_run",
@@ -79,6 +80,9 @@ def lib():
         L.gkm_last_error.argtypes = [P]
         L.gkm_last_error.restype = C.c_char_p
         L.gkm_get_K.argtypes = [P]
+        L.gkm_set_compensation.argtypes = [P, C.c_int]
+        L.gkm_get_compensation.argtypes = [P]
+        L.gkm_download_carry.argtypes = [P, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]
         L.gkm_upload_state_packed.argtypes = [P, _dp]
         L.gkm_download_state_packed.argtypes = [P, _dp]
         L.gkm_upload_state.argtypes = [P, C.POINTER(C.c_void_p)]
@@ -183,6 +187,21 @@ class Context:
     @property
     def K(self):
         return lib().gkm_get_K(self._h)
+
+    @property
+    def compensated(self):
+        """Whether the update carries its rounding error (compensated accumulation)."""
+        return bool(lib().gkm_get_compensation(self._h))
+
+    def set_compensation(self, on):
+        self._chk(lib().gkm_set_compensation(self._h, 1 if on else 0))
+
+    def download_carry(self):
+        """The carried rounding error of the resident state (5, nc, nc, nc) float32,
+        unpadded: the exact running sum is W + carry."""
+        out = np.empty((5, self.nc, self.nc, self.nc), dtype=np.float32)
+        self._chk(lib().gkm_download_carry(self._h, out))
+        return out
 
     def upload(self, W):
         W = np.ascontiguousarray(W, dtype=np.float64)

@@ -17,6 +17,7 @@
 
 #include "../../include/dnsgkm.h"
 #include "adj_kernels.hpp"
+#include "gkm_config.hpp"
 #include "ens_persist.hpp"
 #include "gkm_kernels.hpp"
 #include "lpt_kernels.hpp"
@@ -32,6 +33,12 @@ struct gkm_ctx {
     double* W[2] = {nullptr, nullptr};
     int cur = 0;
     double* dW = nullptr;
+    // compensated accumulation of the update: carried rounding error per cell
+    // and variable (5 x plane floats) of W[0] and W[1]; on by default for
+    // GKM_MIXED (allocated at create), off for GKM_FP64
+    float* cmp[2] = {nullptr, nullptr};
+    bool compensate = false;
+    bool member = false;           // owned by an ensemble (its tables hold the carry pointers)
     double* stage = nullptr;       // 5 * padded
     double* dscal = nullptr;       // [0] dt, [1] vmax
     int* dflag = nullptr;          // non-physical-state flag
@@ -95,6 +102,8 @@ gkm::SweepArgs sweep_args(const gkm_ctx* c, int dir, int mode)
     a.W = c->W[c->cur];
     a.dW = c->dW;
     a.Wout = c->W[1 - c->cur];
+    a.cmp_in = c->compensate ? c->cmp[c->cur] : nullptr;
+    a.cmp_out = c->compensate ? c->cmp[1 - c->cur] : nullptr;
     a.dt = c->dscal;
     a.plane = c->plane;
     a.nc = c->nc;
@@ -147,19 +156,15 @@ int gkm_create(const gkm_config* cfg, gkm_ctx** out)
         *out = c;
         return GKM_EINVAL;
     };
-    if (nc > 1024 || !gkm::sweep_grid_ok(nc)) return bad("nc must be a multiple of 16 in [32, 1024]");
-    if (!(cfg->gam > 1.0) || !(cfg->p0 > 0.0) || !(cfg->rho0 > 0.0) || !(cfg->mu0 >= 0.0))
-        return bad("gam must exceed 1 and p0, rho0 must be positive");
-    if (!(cfg->cfl > 0.0) || !(cfg->prandtl > 0.0)) return bad("cfl and prandtl must be positive");
-    if (cfg->precision != GKM_FP64 && cfg->precision != GKM_MIXED) return bad("precision must be GKM_FP64 or GKM_MIXED");
-    const double pi = std::atan(1.0) * 4.0;
+    {
+        const std::string m = gkm::config_error(*cfg, gkm::sweep_grid_ok);
+        if (!m.empty()) return bad(m);
+    }
     c->nc = nc;
     c->plane = (long)nc * nc * nc;
     c->padded = (long)(nc + 6) * (nc + 6) * (nc + 6);
-    c->dx = 2.0 * pi / nc;                          // FlowField.cpp:46
-    const int K = int((5.0 - 3.0 * cfg->gam) / (cfg->gam - 1.0));  // FlowField.cpp:42
-    const double R = 287.0;                         // param.h:21
-    c->gas = gkm::Gas<double>{cfg->gam, R, 1.0 / cfg->prandtl, cfg->mu0, cfg->p0 / (R * cfg->rho0), 1e-10, K};
+    c->dx = gkm::config_dx(*cfg);
+    c->gas = gkm::config_gas(*cfg);
 
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) return bad(std::string("hipSetDevice: ") + hipGetErrorString(e));
@@ -176,6 +181,14 @@ int gkm_create(const gkm_config* cfg, gkm_ctx** out)
     for (int b = 0; b < 2; b++)
         if (hipMalloc(&c->W[b], sbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc state");
     if (hipMalloc(&c->dW, sbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc dW");
+    if (cfg->precision == GKM_MIXED) {
+        const size_t cbytes = sizeof(float) * 5 * (size_t)c->plane;
+        for (int b = 0; b < 2; b++) {
+            if (hipMalloc(&c->cmp[b], cbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc carry");
+            HIPCHK(c, hipMemsetAsync(c->cmp[b], 0, cbytes, c->stream));
+        }
+        c->compensate = true;
+    }
     if (hipMalloc(&c->stage, sizeof(double) * 5 * (size_t)c->padded) != hipSuccess)
         return fail(c, GKM_ENOMEM, "hipMalloc staging");
     if (hipMalloc(&c->dscal, sizeof(double) * 4) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc scalars");
@@ -197,6 +210,8 @@ void gkm_destroy(gkm_ctx* c)
     (void)hipFree(c->W[0]);
     (void)hipFree(c->W[1]);
     (void)hipFree(c->dW);
+    (void)hipFree(c->cmp[0]);
+    (void)hipFree(c->cmp[1]);
     (void)hipFree(c->stage);
     (void)hipFree(c->dscal);
     (void)hipFree(c->dflag);
@@ -211,6 +226,37 @@ const char* gkm_last_error(const gkm_ctx* c) { return c ? c->err.c_str() : "null
 
 int gkm_get_K(const gkm_ctx* c) { return c ? c->gas.K : -1; }
 
+int gkm_set_compensation(gkm_ctx* c, int on)
+{
+    if (!c) return GKM_EINVAL;
+    if (c->member) return fail(c, GKM_EINVAL, "an ensemble member keeps its precision's compensation setting");
+    if (on && !c->cmp[0]) {
+        const size_t cbytes = sizeof(float) * 5 * (size_t)c->plane;
+        for (int b = 0; b < 2; b++)
+            if (hipMalloc(&c->cmp[b], cbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc carry");
+    }
+    if (on && !c->compensate) {
+        // the resident state starts with no carried error
+        HIPCHK(c, hipMemsetAsync(c->cmp[c->cur], 0, sizeof(float) * 5 * (size_t)c->plane, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->compensate = on != 0;
+    return GKM_OK;
+}
+
+int gkm_get_compensation(const gkm_ctx* c) { return c ? (int)c->compensate : -1; }
+
+int gkm_download_carry(gkm_ctx* c, float* out)
+{
+    if (!c || !out) return fail(c, GKM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
+    if (!c->compensate) return fail(c, GKM_EINVAL, "compensation is off");
+    HIPCHK(c, hipMemcpyAsync(out, c->cmp[c->cur], sizeof(float) * 5 * (size_t)c->plane, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GKM_OK;
+}
+
 int gkm_upload_state(gkm_ctx* c, const double* const W[5])
 {
     if (!c || !W) return fail(c, GKM_EINVAL, "null argument");
@@ -221,6 +267,8 @@ int gkm_upload_state(gkm_ctx* c, const double* const W[5])
     }
     int rc = timed(c, 7, [&] { return gkm::launch_unpack(c->stage, c->W[c->cur], c->nc, c->stream); });
     if (rc) return rc;
+    if (c->cmp[c->cur])  // a new state carries no rounding error
+        HIPCHK(c, hipMemsetAsync(c->cmp[c->cur], 0, sizeof(float) * 5 * (size_t)c->plane, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     c->lpt.valid = false;  // a spline built earlier described the previous state
@@ -995,6 +1043,8 @@ gkm::SweepArgs ens_sweep_args(const gkm_ctx* c, int p, int d, int v)
     a.W = c->W[p];
     a.dW = c->dW;
     a.Wout = c->W[1 - p];
+    a.cmp_in = c->compensate ? c->cmp[p] : nullptr;
+    a.cmp_out = c->compensate ? c->cmp[1 - p] : nullptr;
     a.dt = c->dscal;
     a.plane = c->plane;
     a.nc = c->nc;
@@ -1247,6 +1297,7 @@ int gkm_ensemble_create(int n, const gkm_config* cfgs, gkm_ensemble** out)
             gkm_destroy(c);
             return efail(e, rc, "member " + std::to_string(m) + ": " + msg);
         }
+        c->member = true;
         e->mem.push_back(c);
     }
     EHIPCHK(e, hipSetDevice(cfgs[0].device));

@@ -12,6 +12,12 @@ struct SweepArgs {
     const double* W;     // state at t^n
     double* dW;          // sum of flux differences of the sweeps done so far
     double* Wout;        // state at t^{n+1} (written by the last sweep)
+    // Compensated accumulation of the update (mode 2), or nullptr for the plain
+    // FP64 update: cmp_in holds, per cell and variable, the rounding error the
+    // previous step's update left out of W (exact sum = W + cmp_in); the update
+    // adds it to the increment and stores its own error in cmp_out (TwoSum).
+    const float* cmp_in;
+    float* cmp_out;
     const double* dt;    // device scalar (graph-capturable)
     long plane;          // nc^3
     int nc;

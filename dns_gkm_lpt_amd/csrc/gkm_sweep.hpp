@@ -256,7 +256,18 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
                 const long o = q * plane + idx;
                 if (p.mode == 0) p.dW[o] = dF;
                 else if (p.mode == 1) p.dW[o] += dF;
-                else p.Wout[o] = W[o] - p.inv_dx * (p.dW[o] + dF);
+                else if (p.cmp_out) {
+                    // W^{n+1} + e = W^n + (increment + carried error), exactly (TwoSum)
+                    const double w = W[o];
+                    const double y = (double)p.cmp_in[o] - p.inv_dx * (p.dW[o] + dF);
+                    const double t = w + y;
+                    const double yv = t - w;
+                    const double e = (w - (t - yv)) + (y - yv);
+                    p.Wout[o] = t;
+                    p.cmp_out[o] = (float)e;
+                } else {
+                    p.Wout[o] = W[o] - p.inv_dx * (p.dW[o] + dF);
+                }
             }
         }
         __syncthreads();

@@ -73,6 +73,19 @@ const char *gkm_last_error(const gkm_ctx *ctx);
 /* internal degrees of freedom K = int((5-3gam)/(gam-1)) (FlowField.cpp:42) */
 int gkm_get_K(const gkm_ctx *ctx);
 
+/* Compensated accumulation of the update W^{n+1} = W^n - (dFx + dFy + dFz)/dx
+ * (update.cpp:505-516; no reference counterpart): with it on, each cell keeps
+ * the rounding error of its last FP64 update in a per-variable FP32 carry, adds
+ * it to the next increment and stores the new error (TwoSum), so W + carry is
+ * the exact running sum of the rounded increments.  On by default for
+ * GKM_MIXED, off for GKM_FP64 (whose update is the reference's).  Uploading a
+ * state clears the carry.  gkm_set_compensation fails on an ensemble member.
+ * gkm_download_carry: the carry of the resident state, 5 x nc^3 floats,
+ * unpadded, (y, x, z) with z fastest. */
+int gkm_set_compensation(gkm_ctx *ctx, int on);
+int gkm_get_compensation(const gkm_ctx *ctx);
+int gkm_download_carry(gkm_ctx *ctx, float *out);
+
 /* Host -> device state, padded layout, 5 separate arrays (Initialize_DHIT's
  * W[5], FlowField.cpp:60-114; HtoD of GPUupdate.cu:89-93). Ghosts are ignored. */
 int gkm_upload_state(gkm_ctx *ctx, const double *const W[5]);

@@ -11,9 +11,15 @@ from the device CFL after every step and t += dt (the reference's order).
 This test process never touches the GPU.  The oracle replays the same loop
 on the CPU from the same files.
 
-Tolerances (the FP64 32^3 run's): statistics lines 1e-9 relative per value
-(t / t0 1e-12); P_ files: positions 1e-9 (minimum-image distance), every
-property column 1e-9 of its largest magnitude; T_ files 1e-12; the
+Tolerances (the FP64 32^3 run's): statistics lines at their print precision
+(the reference prints 9 significant digits, std::scientific with precision 8:
+1e-8 relative per value, plus 1e-12 absolute for the normalised P_rms and
+T_rms, which are zero up to rounding at iteration 0 where p and T are
+uniform, and for t / t0, which is 0 there); P_ files (printed with 16
+significant digits): positions 1e-9 (minimum-image distance), every property
+column 1e-9 of its largest magnitude over iterations 0, 10 and 20 (at
+iteration 0 rho, p and T are uniform, so their gradients and the pressure
+Hessian are zero up to rounding there); T_ files 1e-12; the
 iteration-0 field and restart files equal the initial condition to print
 precision (1e-14 relative).
 """
@@ -103,19 +109,21 @@ def test_driver_run_matches_oracle(tmp_path):
     assert rows.shape == (3, 16), rows.shape
     for r, it in zip(rows, (0, 10, 20)):
         ref = rec[it]["line"]
-        assert abs(r[0] - ref[0]) <= 1e-12 * max(abs(ref[0]), 1e-300)
-        err = np.abs(r[1:] - ref[1:]) / np.maximum(np.abs(ref[1:]), 1e-300)
-        print(f"dropin: stats line {it}: worst relative error {err.max():.2e}", flush=True)
-        assert np.all(err <= 1e-9), (it, err)
+        assert abs(r[0] - ref[0]) <= 1e-8 * abs(ref[0]) + 1e-12
+        d = np.abs(r[1:] - ref[1:])
+        print(f"dropin: stats line {it}: worst relative error "
+              f"{np.max(d / np.maximum(np.abs(ref[1:]), 1e-300)):.2e}", flush=True)
+        assert np.all(d <= 1e-8 * np.abs(ref[1:]) + 1e-12), (it, r, ref)
 
+    cols = [c for c in outm.P_COLUMNS if not isinstance(c, tuple)]
+    scale = np.max([np.max(np.abs(rec[it]["props"][:, cols]), axis=0) for it in (0, 10, 20)], axis=0)
     for it in (0, 10, 20):
         tags, pos, props = read_p_file(str(tmp_path / "Results" / "Particles" / f"P_0_{it}"))
         assert np.array_equal(tags, np.arange(NPART))
         d = np.abs(pos - rec[it]["pos"]) % (2 * np.pi)
         assert np.max(np.minimum(d, 2 * np.pi - d)) <= 1e-9
-        cols = [c for c in outm.P_COLUMNS if not isinstance(c, tuple)]
         ref = rec[it]["props"][:, cols]
-        err = np.max(np.abs(props[:, cols] - ref), axis=0) / np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
+        err = np.max(np.abs(props[:, cols] - ref), axis=0) / scale
         print(f"dropin: P_0_{it}: worst column error {err.max():.2e}", flush=True)
         assert np.all(err <= 1e-9), (it, err)
         tt = np.array([float(v) for v in open(tmp_path / "Results" / "Time" / f"T_{it}").read().split()])

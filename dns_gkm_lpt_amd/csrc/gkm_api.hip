@@ -112,7 +112,7 @@ gkm::SweepArgs sweep_args(const gkm_ctx* c, int dir, int mode)
     a.g = c->gas;
     a.tile = c->tile[dir];
     a.seg = gkm::sweep_seg_len(dir, c->nc, a.tile, c->slots);
-    a.nseg = c->nc / a.seg;
+    a.nseg = gkm::sweep_nseg(c->nc, a.seg);
     a.mode = mode;
     return a;
 }
@@ -165,6 +165,7 @@ int gkm_create(const gkm_config* cfg, gkm_ctx** out)
     c->padded = (long)(nc + 6) * (nc + 6) * (nc + 6);
     c->dx = gkm::config_dx(*cfg);
     c->gas = gkm::config_gas(*cfg);
+    for (int d = 0; d < 3; d++) c->tile[d] = gkm::sweep_first_tile(d, nc);
 
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) return bad(std::string("hipSetDevice: ") + hipGetErrorString(e));
@@ -476,7 +477,8 @@ int gkm_set_sweep_tiles(gkm_ctx* c, const int tile[3])
 {
     if (!c || !tile) return fail(c, GKM_EINVAL, "null argument");
     for (int d = 0; d < 3; d++)
-        if (tile[d] < 0 || tile[d] >= gkm::kTileVariants) return fail(c, GKM_EINVAL, "tile variant out of range");
+        if (!gkm::sweep_tile_ok(d, tile[d], c->nc))
+            return fail(c, GKM_EINVAL, "tile variant out of range or larger than the grid");
     for (int d = 0; d < 3; d++) c->tile[d] = tile[d];
     return GKM_OK;
 }
@@ -500,10 +502,14 @@ int gkm_autotune(gkm_ctx* c, int reps, int tile[3], double ms[9])
         double best = 0.0;
         int best_v = c->tile[dir];
         for (int v = 0; v < gkm::kTileVariants; v++) {
+            if (!gkm::sweep_tile_ok(dir, v, c->nc)) {
+                if (ms) ms[3 * dir + v] = -1.0;
+                continue;
+            }
             gkm::SweepArgs a = sweep_args(c, dir, dir);
             a.tile = v;
             a.seg = gkm::sweep_seg_len(dir, c->nc, v, c->slots);
-            a.nseg = c->nc / a.seg;
+            a.nseg = gkm::sweep_nseg(c->nc, a.seg);
             // dW is scratch and Wout is the next state's buffer: the resident state is not changed
             hipError_t e = gkm::launch_sweep(dir, a, mixed, c->stream);  // warm-up
             if (e != hipSuccess) return fail(c, GKM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -518,7 +524,7 @@ int gkm_autotune(gkm_ctx* c, int reps, int tile[3], double ms[9])
             HIPCHK(c, hipEventElapsedTime(&el, c->ev[0], c->ev[1]));
             const double avg = el / reps;
             if (ms) ms[3 * dir + v] = avg;
-            if (v == 0 || avg < best) {
+            if (best == 0.0 || avg < best) {
                 best = avg;
                 best_v = v;
             }
@@ -1053,7 +1059,7 @@ gkm::SweepArgs ens_sweep_args(const gkm_ctx* c, int p, int d, int v)
     a.g = c->gas;
     a.tile = v;
     a.seg = gkm::sweep_seg_len(d, c->nc, v, 0);
-    a.nseg = c->nc / a.seg;
+    a.nseg = gkm::sweep_nseg(c->nc, a.seg);
     a.mode = d;
     return a;
 }
@@ -1309,7 +1315,16 @@ int gkm_ensemble_create(int n, const gkm_config* cfgs, gkm_ensemble** out)
         hipMalloc(&e->dcfl, sizeof(gkm::CflArgs) * 2 * (size_t)n) != hipSuccess ||
         hipMalloc(&e->partial, sizeof(double) * gkm::kEnsCflBlocks * (size_t)n) != hipSuccess)
         return efail(e, GKM_ENOMEM, "hipMalloc ensemble tables");
-    const int t0[3] = {0, 0, 0};
+    int t0[3];
+    for (int d = 0; d < 3; d++) {
+        t0[d] = -1;
+        for (int v = 0; v < gkm::kTileVariants && t0[d] < 0; v++) {
+            bool ok = true;
+            for (gkm_ctx* c : e->mem) ok = ok && gkm::sweep_tile_ok(d, v, c->nc);
+            if (ok) t0[d] = v;
+        }
+        if (t0[d] < 0) return efail(e, GKM_EINVAL, "no sweep tile fits every member's grid");
+    }
     return ens_build(e, t0);
 }
 
@@ -1344,7 +1359,9 @@ int gkm_ensemble_set_tiles(gkm_ensemble* e, const int tile[3])
 {
     if (!e || !tile) return efail(e, GKM_EINVAL, "null argument");
     for (int d = 0; d < 3; d++)
-        if (tile[d] < 0 || tile[d] >= gkm::kTileVariants) return efail(e, GKM_EINVAL, "tile variant out of range");
+        for (gkm_ctx* c : e->mem)
+            if (!gkm::sweep_tile_ok(d, tile[d], c->nc))
+                return efail(e, GKM_EINVAL, "tile variant out of range or larger than a member's grid");
     return ens_build(e, tile);
 }
 
@@ -1360,7 +1377,14 @@ int gkm_ensemble_autotune(gkm_ensemble* e, int reps, int tile[3], double ms[9])
     for (int d = 0; d < 3; d++) {
         double best_ms = 0.0;
         best[d] = e->tile[d];
+        bool first = true;
         for (int v = 0; v < gkm::kTileVariants; v++) {
+            bool ok = true;
+            for (gkm_ctx* c : e->mem) ok = ok && gkm::sweep_tile_ok(d, v, c->nc);
+            if (!ok) {
+                if (ms) ms[3 * d + v] = -1.0;
+                continue;
+            }
             int t[3] = {e->tile[0], e->tile[1], e->tile[2]};
             t[d] = v;
             rc = ens_build(e, t);
@@ -1379,9 +1403,10 @@ int gkm_ensemble_autotune(gkm_ensemble* e, int reps, int tile[3], double ms[9])
             EHIPCHK(e, hipEventElapsedTime(&el, e->ev[0], e->ev[1]));
             const double avg = el / reps;
             if (ms) ms[3 * d + v] = avg;
-            if (v == 0 || avg < best_ms) {
+            if (first || avg < best_ms) {
                 best_ms = avg;
                 best[d] = v;
+                first = false;
             }
         }
         int t[3] = {e->tile[0], e->tile[1], e->tile[2]};

@@ -12,7 +12,7 @@ namespace gkm {
 // empty when cfg is usable; otherwise what is wrong with it
 inline std::string config_error(const gkm_config& cfg, bool (*grid_ok)(int))
 {
-    if (cfg.nc > 1024 || !grid_ok(cfg.nc)) return "nc must be a multiple of 16 in [32, 1024]";
+    if (cfg.nc > 1024 || !grid_ok(cfg.nc)) return "nc must be in [16, 1024]";
     if (!(cfg.gam > 1.0) || !(cfg.p0 > 0.0) || !(cfg.rho0 > 0.0) || !(cfg.mu0 >= 0.0))
         return "gam must exceed 1 and p0, rho0 must be positive";
     if (!(cfg.cfl > 0.0) || !(cfg.prandtl > 0.0)) return "cfl and prandtl must be positive";

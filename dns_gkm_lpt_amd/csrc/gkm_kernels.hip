@@ -39,43 +39,67 @@ __global__ __launch_bounds__(256, 2) void ens_sweep_kernel(const SweepArgs* __re
     sweep_body<DIR, MIX, V>(p, g - first[lo], first[lo + 1] - first[lo], false);
 }
 
+static int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+bool sweep_tile_ok(int dir, int tile, int nc)
+{
+    // Any grid size: the pencils and segments of the last tiles reach past nc
+    // (those lines and faces are computed with wrapped cells and not written).
+    // Every wrapped index must stay in [-nc, 2nc), the range wrapi handles:
+    // stencils reach CN + 2 cells before a segment and past the grid's end,
+    // pencils one line past a tile.
+    if (dir < 0 || dir > 2 || tile < 0 || tile >= kTileVariants) return false;
+    const int CN = kTiles[dir][tile][0], CA = kTiles[dir][tile][1], CB = kTiles[dir][tile][2];
+    return CN + 3 <= nc && CA <= nc && CB <= nc;
+}
+
+int sweep_first_tile(int dir, int nc)
+{
+    for (int v = 0; v < kTileVariants; v++)
+        if (sweep_tile_ok(dir, v, nc)) return v;
+    return -1;
+}
+
 bool sweep_grid_ok(int nc)
 {
-    // multiples of 16 fit every tile shape; nc >= 32 keeps every wrapped index
-    // (faces of the lead-in chunk reach 16 + 2 cells before the segment) in
-    // [-nc, 2nc), the range wrapi handles.
-    return nc >= 32 && nc % 16 == 0;
+    return nc >= 16 && sweep_first_tile(0, nc) >= 0 && sweep_first_tile(1, nc) >= 0 && sweep_first_tile(2, nc) >= 0;
 }
 
 int sweep_seg_len(int dir, int nc, int tile, int slots)
 {
-    // faces per workgroup along the sweep axis; a segment costs seg/CN chunks
-    // plus one lead-in chunk (the face before the segment).
-    if (dir < 0 || dir > 2 || tile < 0 || tile >= kTileVariants) return 0;
+    // faces per workgroup along the sweep axis (a multiple of CN); a segment
+    // costs seg/CN chunks plus one lead-in chunk (the face before the segment).
+    if (!sweep_tile_ok(dir, tile, nc)) return 0;
     const int CN = kTiles[dir][tile][0], CA = kTiles[dir][tile][1], CB = kTiles[dir][tile][2];
-    const long pencils = (long)(nc / CA) * (nc / CB);
+    const long pencils = (long)cdiv(nc, CA) * cdiv(nc, CB);
+    const int lchunks = cdiv(nc, CN);  // chunks of a whole line
     if (slots > 0) {
         // one grid: the length with the fewest chunk-times, counting rounds of
         // `slots` concurrent workgroups; the longer segment on ties
-        int best = nc;
+        int best = lchunks * CN;
         long best_cost = -1;
-        for (int seg = nc; seg >= CN && seg % CN == 0 && nc % seg == 0; seg /= 2) {
-            const long blocks = pencils * (nc / seg);
+        for (int parts = 1;; parts *= 2) {
+            const int seg = cdiv(lchunks, parts) * CN;
+            const long blocks = pencils * cdiv(nc, seg);
             const long cost = (blocks + slots - 1) / slots * (seg / CN + 1);
             if (best_cost < 0 || cost < best_cost) {
                 best_cost = cost;
                 best = seg;
             }
-            if (seg % 2) break;
+            if (seg == CN) break;
         }
         return best;
     }
     // members of an ensemble (their workgroups add up): the whole line when
-    // that already gives >= 1024 workgroups (4 per CU), otherwise split while
+    // that already gives >= 1024 workgroups (4 per CU), otherwise halve while
     // segments stay >= 8 chunks and >= 64 faces long.
     const int min_seg = 8 * CN > 64 ? 8 * CN : 64;
-    int seg = nc;
-    while (pencils * (nc / seg) < 1024 && seg % 2 == 0 && (seg / 2) % CN == 0 && seg / 2 >= min_seg) seg /= 2;
+    int seg = lchunks * CN;
+    while (pencils * cdiv(nc, seg) < 1024) {
+        const int half = cdiv(seg / CN, 2) * CN;
+        if (half >= seg || half < min_seg) break;
+        seg = half;
+    }
     return seg;
 }
 
@@ -83,8 +107,9 @@ template <int DIR, int V>
 static hipError_t launch_tile(const SweepArgs& a, bool mixed, hipStream_t s)
 {
     constexpr int CN = Tile<DIR, V>::CN, CA = Tile<DIR, V>::CA, CB = Tile<DIR, V>::CB;
-    if (a.nc % CA || a.nc % CB || a.seg % CN || a.nc % a.seg || a.nseg != a.nc / a.seg) return hipErrorInvalidValue;
-    const long nblk = (long)(a.nc / CA) * (a.nc / CB) * a.nseg;
+    if (!sweep_tile_ok(DIR, V, a.nc) || a.seg <= 0 || a.seg % CN || a.nseg != cdiv(a.nc, a.seg))
+        return hipErrorInvalidValue;
+    const long nblk = (long)cdiv(a.nc, CA) * cdiv(a.nc, CB) * a.nseg;
     if (mixed)
         hipLaunchKernelGGL((sweep_kernel<DIR, true, V>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
     else
@@ -114,9 +139,9 @@ hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s)
 
 int sweep_blocks(int dir, int nc, int tile, int seg)
 {
-    if (dir < 0 || dir > 2 || tile < 0 || tile >= kTileVariants || seg <= 0) return 0;
+    if (!sweep_tile_ok(dir, tile, nc) || seg <= 0) return 0;
     const int CA = kTiles[dir][tile][1], CB = kTiles[dir][tile][2];
-    return (nc / CA) * (nc / CB) * (nc / seg);
+    return cdiv(nc, CA) * cdiv(nc, CB) * cdiv(nc, seg);
 }
 
 template <int DIR, int V>

@@ -55,7 +55,13 @@ hipError_t launch_ens_sweep(int dir, int tile, const SweepArgs* members, const i
 // faces per workgroup chosen for a grid: slots = concurrent workgroups of the
 // device for a single grid, 0 for the members of an ensemble
 int sweep_seg_len(int dir, int nc, int tile, int slots);
-bool sweep_grid_ok(int nc);                    // nc compatible with every tile shape
+// any nc in [16, 1024] has a tile variant per direction; a variant fits nc when
+// CN + 3 <= nc and CA, CB <= nc
+bool sweep_grid_ok(int nc);
+bool sweep_tile_ok(int dir, int tile, int nc);
+int sweep_first_tile(int dir, int nc);           // the first variant that fits, or -1
+// segments along the sweep axis: nseg = ceil(nc / seg); the last may reach past nc
+inline int sweep_nseg(int nc, int seg) { return seg > 0 ? (nc + seg - 1) / seg : 0; }
 
 // CFL: partial maxima (one per block) then a one-block final pass writing
 // dt = cfl*dx/max and a non-physical-state flag.

@@ -157,7 +157,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
     if (remap && (nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
     const int s = bid % p.nseg;
     const int pencil = bid / p.nseg;
-    const int npb = nc / CB;
+    const int npb = (nc + CB - 1) / CB;  // pencils across b; the last ones may reach past nc
     const int a0 = (pencil / npb) * CA;
     const int b0 = (pencil % npb) * CB;
 
@@ -173,11 +173,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
         tn = tid / (CA * CB);
     }
     const int lc = ta * CB + tb;
+    // a grid that is not a multiple of the tile: lines past nc (their cells
+    // wrap) and faces past the segment's end are computed, not written
+    const bool live = a0 + ta < nc && b0 + tb < nc;
 
     const double dt = *p.dt;
     const double dx = p.dx;
     const int nchunk = p.seg / CN;
     const int seg0 = s * p.seg;
+    const int seg_end = seg0 + p.seg < nc ? seg0 + p.seg : nc;
 
     SweepIn<DIR, CN, L> in;
     in.W = W;
@@ -227,7 +231,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
         // ---- one GKS flux per thread
         {
             const int fn = n0 + tn;
-            const int ca = a0 + ta, cb = b0 + tb;  // interior lines: in range
+            const int ca = wrapi(a0 + ta, nc), cb = wrapi(b0 + tb, nc);
             in.iL = cell_index<DIR>(wrapi(fn, nc), ca, cb, nc);
             in.iR = cell_index<DIR>(wrapi(fn + 1, nc), ca, cb, nc);
             double tau;
@@ -248,7 +252,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
         __syncthreads();
 
         // ---- flux differences of the chunk's cells
-        if (c >= 0) {
+        if (c >= 0 && live && n0 + tn < seg_end) {
             const long idx = cell_index<DIR>(n0 + tn, a0 + ta, b0 + tb, nc);
 #pragma unroll
             for (int q = 0; q < 5; q++) {

@@ -135,8 +135,8 @@ def test_reference_drop_of_tracers_at_two_pi(drop):
     the reference's single-rank run pops it (update.cpp:1056-1109); a tracer
     uploaded at x = 2 pi is not located on rank 0 (update.cpp:1304-1316).
     Without the mode both are kept on the face x = 2 pi."""
-    nc = 32
-    z = np.zeros((nc + 6,) * 3)  # the host layout: 3 ghost layers per face
+    nc = 16
+    z = np.zeros((nc,) * 3)
     u, v = -1e-15, 3.0
     W = icm.prim_to_cons(z + u, z + v, z, z + 1.1, z + 86100.0)
     P = {"gam": 1.4, "rho0": 1.0, "Mt": 0.3, "Re": 30.0, "mu0": 1e-3, "p0": 86100.0, "t0": 1.0}

@@ -125,7 +125,7 @@ def test_conservation_to_roundoff():
 
 def test_invalid_config_rejected():
     W, P, gas = make_case(32)
-    for bad in (20, 16, 2048):
+    for bad in (8, 15, 1025, 2048):  # any nc in [16, 1024] is accepted
         with pytest.raises(gl.GkmError) as ei:
             gl.Context(bad, P)
         assert ei.value.code == 1

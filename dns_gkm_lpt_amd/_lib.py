@@ -20,7 +20,7 @@ EXPORTS = [
     "gkm_create", "gkm_destroy", "gkm_last_error", "gkm_get_K", "gkm_set_compensation", "gkm_get_compensation",
     "gkm_download_carry", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
-    "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune",
+    "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune", "gkm_step_noalloc",
     "gkm_tl_run", "gkm_adjoint_run",
     "gkm_ensemble_create", "gkm_ensemble_destroy", "gkm_ensemble_last_error", "gkm_ensemble_member",
     "gkm_ensemble_set_tiles", "gkm_ensemble_autotune", "gkm_ensemble_run", "gkm_ensemble_dt",
@@ -81,6 +81,8 @@ def lib():
         L.gkm_last_error.restype = C.c_char_p
         L.gkm_get_K.argtypes = [P]
         L.gkm_set_compensation.argtypes = [P, C.c_int]
+        L.gkm_step_noalloc.argtypes = [C.POINTER(Config), C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                       C.c_char_p, C.c_int]
         L.gkm_get_compensation.argtypes = [P]
         L.gkm_download_carry.argtypes = [P, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]
         L.gkm_upload_state_packed.argtypes = [P, _dp]

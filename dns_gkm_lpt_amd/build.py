@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdnsgkm.so")
 SOURCES = ["gkm_kernels.hip", "ens_persist.hip", "lpt_kernels.hip", "lpt_sort.hip", "adj_kernels.hip",
-           "gkm_api.hip"]
+           "gkm_api.hip", "gkm_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

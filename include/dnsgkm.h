@@ -125,6 +125,21 @@ int gkm_get_sweep_tiles(const gkm_ctx *ctx, int tile[3]);
 int gkm_autotune(gkm_ctx *ctx, int reps, int tile[3], double ms[9]);
 /* Wait for all work on the context's stream. */
 int gkm_sync(gkm_ctx *ctx);
+/* One Eulerian step of size dt on caller-owned device buffers, allocating no
+ * device memory: W and Wout are device arrays of 5 x nc^3 doubles in the
+ * library's device layout (rho, rho u, rho v, rho w, rho E planes, cell
+ * (y, x, z) at (y nc + x) nc + z, no ghost layers), W != Wout; stream is a
+ * hipStream_t (NULL: the null stream).  Results equal gkm_step's on a context
+ * with the same configuration and default tiles (for GKM_MIXED, with
+ * compensation off: this step does the plain FP64 update).  No reference
+ * counterpart: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops).  Its scratch (the flux-difference accumulator and
+ * the device dt) is one file-scope device array per GPU sized for nc <= 64;
+ * a larger nc is refused with GKM_EINVAL.  Calls that overlap on one GPU race
+ * on that array.  msg (may be NULL) receives an error message of at most
+ * msglen bytes. */
+int gkm_step_noalloc(const gkm_config *cfg, const double *W, double *Wout, double dt, void *stream, char *msg,
+                     int msglen);
 /* Flow statistics of the resident state (PrintStats, update.cpp:688-842). */
 int gkm_stats_compute(gkm_ctx *ctx, gkm_stats *out);
 

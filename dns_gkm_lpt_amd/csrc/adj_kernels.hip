@@ -9,19 +9,23 @@
 // numbers (gkm_dual.hpp):
 //   tangent-linear: one dual pass per face with the perturbation as tangent;
 //   adjoint: per face, the flux's 50 partials contracted with the face's
-//   adjoint flux (face_flux_adjoint: one reverse pass for the inputs in which
-//   the flux is affine, 11 dual passes for the rest); the slopes are linear in
-//   the WENO states, so their adjoint is an explicit gather; per face and line, WENO in
-//   reverse mode (one pass per one-dimensional reconstruction, the energy slot
-//   by the chain rule); then a gather per cell.
-// One thread per face or cell; these kernels favour clarity over speed (they
-// serve the 64^3 adjoint configuration).
+//   adjoint flux (face_flux_vjp, adj_flux.hpp: one forward and one reverse pass
+//   through the whole flux); the slopes are linear in the WENO states, so
+//   their adjoint is an explicit gather; per face and line, WENO in reverse
+//   mode (one pass per one-dimensional reconstruction, the energy slot by the
+//   chain rule); then a gather per cell.
+// One thread per face or cell (they serve the 64^3 adjoint configuration).
+// The flux adjoint runs one workgroup per CU (launch bounds 256, 1): with 512
+// registers per lane and its side blocks in a function of their own it spills
+// least, and at 64^3 that measured fastest (0.60 s per 200-step gradient,
+// against 0.78 s inlined and 0.89 s at two workgroups per CU).
 //
 // Face arrays hold one value per face and direction at the index of the face's
 // left cell.  Direction d: sweep axis n, tangential axes a, b as in the sweep
 // kernels (x: n = x, a = y, b = z; y: n = y, a = x, b = z; z: n = z, a = x, b = y).
 #include "adj_kernels.hpp"
 
+#include "adj_flux.hpp"
 #include "adj_math.hpp"
 #include "gkm_dual.hpp"
 #include "gkm_flux.hpp"
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256, 2) void tl_update_kernel(const double* __restr
 // Xbar[k][f] = sum_q Fbar_q dF_q / dx_k for the 50 flux inputs of face f, with
 // Fbar = -(1/dx) (lam(cell n) - lam(cell n+1)); stored by input group and lab slot.
 template <int DIR>
-__global__ __launch_bounds__(256, 2) void flux_adj_kernel(const double* __restrict__ W, const double* __restrict__ WS,
+__global__ __launch_bounds__(256, 1) void flux_adj_kernel(const double* __restrict__ W, const double* __restrict__ WS,
                                                        const double* __restrict__ lam, long plane, int nc,
                                                        Gas<double> g, double dt, double dx,
                                                        double* __restrict__ Xbar)
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(256, 2) void flux_adj_kernel(const double* __restri
     double x[50];
     gather_inputs<DIR, double>(n, a, b, nc, plane, dx, get, x);
     double xbar[50];
-    face_flux_adjoint(x, dt, dx, g, Fbar, xbar);
+    face_flux_vjp(x, dt, dx, g, Fbar, xbar);
     for (int k = 0; k < 50; k++) Xbar[(long)(5 * (k / 5) + phys<DIR>(k % 5)) * plane + f] = xbar[k];
 }
 

@@ -132,220 +132,6 @@ __host__ __device__ __forceinline__ void mom_uvw_T(const double* H, double th, d
     ab[4] += t * accb[c];
 }
 
-// Adjoint of the face flux with respect to the inputs in which it is affine:
-// for gks_flux_tf<double, double> with the WENO states Wl, Wr and the collision
-// time tau (of the left cell) held fixed, the flux is affine in the six slope
-// vectors D and in the cell states WL, WR (these enter only through the
-// equilibrium state's normal slopes (We - WL) 2/dx and (WR - We) 2/dx).  Gives
-// Dbar[d][q] = sum_k Fbar[k] dF_k/dD_d[q] and WLbar, WRbar likewise, at fixed tau.
-// The prelude repeats gks_flux_tf's (states, time coefficients, half-space
-// tables, equilibrium state); the affine terms are then walked backwards,
-// building block by building block, through their transposes.
-__host__ __device__ inline void flux_affine_adjoint(const double Wl[5], const double Wr[5], double tau, double dt,
-                                                    double dx, const Gas<double>& g, const double Fbar[5],
-                                                    double WLbar[5], double WRbar[5], double Dbar[6][5])
-{
-    const double pi = 3.141592653589793238462643383279502884;
-    const double kp2 = double(g.K + 2), kp3 = double(g.K + 3), kp4 = double(g.K + 4);
-    const double ikp3 = 1.0 / kp3, inv_prn = g.inv_prn;
-    Maxw<double> L, R;
-    maxw_init(L, Wl, g);
-    maxw_init(R, Wr, g);
-    {
-        const double pl = L.den / L.lam, pr = R.den / R.lam;
-        tau = tau + dt * (::fabs(pl - pr) / ::fabs(pl + pr));
-    }
-    const double ex = g_exp(-dt / tau);
-    const double om = 1.0 - ex;
-    const double igm0 = 1.0 / (dt - tau * om);
-    const double gm1 = -om * igm0;
-    const double gm2 = (-dt + 2.0 * tau * om - dt * ex) * igm0;
-    const double gm3 = -gm1;
-    const double gm4 = (dt * ex - tau * om) * igm0;
-    const double gm5 = tau * gm3;
-    const double Pc = tau * om;
-    const double Qc = -tau * dt * ex + tau * tau * om;
-    const double p0 = dt - Pc;
-    const double p1 = 0.5 * dt * dt - tau * p0;
-    const double p2 = -tau * p0 + Qc;
-    const double p4 = -Qc - tau * Pc;
-    const double p5 = -tau * Pc;
-    double hl[4], hr[4];
-    {
-        const double gl = 0.5 * g_exp(-L.lam * L.u * L.u) / g_sqrt(L.lam * pi);
-        const double h0 = 0.5 * g_erfc(-g_sqrt(L.lam) * L.u);
-        half_tab<4>(L.u, L.i2l, h0, L.u * h0 + gl, hl);
-        const double gr = 0.5 * g_exp(-R.lam * R.u * R.u) / g_sqrt(R.lam * pi);
-        const double r0 = 0.5 * g_erfc(g_sqrt(R.lam) * R.u);
-        half_tab<4>(R.u, R.i2l, r0, R.u * r0 - gr, hr);
-    }
-    double We[5];
-    {
-        double m[5] = {hl[0], hl[1], 0.0, 0.0, 0.5 * (hl[2] + kp2 * L.i2l * hl[0])};
-        tshift(m, L.v, L.w);
-        for (int q = 0; q < 5; q++) We[q] = L.den * m[q];
-        double n[5] = {hr[0], hr[1], 0.0, 0.0, 0.5 * (hr[2] + kp2 * R.i2l * hr[0])};
-        tshift(n, R.v, R.w);
-        for (int q = 0; q < 5; q++) We[q] += R.den * n[q];
-    }
-    Maxw<double> E;
-    maxw_init(E, We, g);
-
-    // ---- reverse: F = F64 + tshift(Fs, E.v, E.w)
-    double Fs_b[5];
-    for (int q = 0; q < 5; q++) Fs_b[q] = Fbar[q];
-    tshift_T(Fs_b, E.v, E.w);
-    // equilibrium tangential and time-slope terms
-    const TState<double> sE{E.u, E.i2l, 2.0 * E.lam};
-    double fuE[6];
-    full_tab<6>(sE.u, sE.th, fuE);
-    const double cBE = gm2 * E.den, cFE = p2 * E.den, c1 = p1 * E.den, idenE = 1.0 / E.den;
-    double t_b[5] = {0, 0, 0, 0, 0}, tvwE_b[5], aE_b[5], Be_b[5], ayE_b[5] = {0, 0, 0, 0, 0},
-           azE_b[5] = {0, 0, 0, 0, 0};
-    mom_u_T(fuE + 1, sE.th, kp2, kp4, Fs_b, t_b);  // Fs += mom_u(fu + 1, cF tvw + c1 a)
-    for (int q = 0; q < 5; q++) {
-        tvwE_b[q] = cFE * t_b[q];
-        aE_b[q] = c1 * t_b[q];
-    }
-    slope_t_T(aE_b, sE, inv_prn, kp3, ikp3, Be_b);  // a = slope_t(Be / den)
-    for (int q = 0; q < 5; q++) Be_b[q] *= idenE;
-    {
-        double tmp[5] = {0, 0, 0, 0, 0};
-        mom_uvw_T(fuE + 1, sE.th, kp4, Fs_b, tmp, 3);  // Fs += mom_uvw(fu + 1, cF az, 3)
-        for (int q = 0; q < 5; q++) azE_b[q] += cFE * tmp[q];
-        for (int q = 0; q < 5; q++) tmp[q] = 0.0;
-        mom_uvw_T(fuE + 1, sE.th, kp4, Fs_b, tmp, 2);  // Fs += mom_uvw(fu + 1, cF ay, 2)
-        for (int q = 0; q < 5; q++) ayE_b[q] += cFE * tmp[q];
-        for (int q = 0; q < 5; q++) tmp[q] = 0.0;
-        mom_u_T(fuE, sE.th, kp2, kp4, Be_b, tmp);  // Be += mom_u(fu, cB tvw)
-        for (int q = 0; q < 5; q++) tvwE_b[q] += cBE * tmp[q];
-        for (int q = 0; q < 5; q++) tmp[q] = 0.0;
-        mom_uvw_T(fuE, sE.th, kp4, Be_b, tmp, 3);  // Be += mom_uvw(fu, cB az, 3)
-        for (int q = 0; q < 5; q++) azE_b[q] += cBE * tmp[q];
-        for (int q = 0; q < 5; q++) tmp[q] = 0.0;
-        mom_uvw_T(fuE, sE.th, kp4, Be_b, tmp, 2);  // Be += mom_uvw(fu, cB ay, 2)
-        for (int q = 0; q < 5; q++) ayE_b[q] += cBE * tmp[q];
-    }
-    for (int q = 0; q < 5; q++) {  // tvw = V ay + W az
-        ayE_b[q] += E.v * tvwE_b[q];
-        azE_b[q] += E.w * tvwE_b[q];
-    }
-    double bye_b[5], bze_b[5];
-    slope_t_T(ayE_b, sE, inv_prn, kp3, ikp3, bye_b);  // ay = slope_t(bye)
-    slope_t_T(azE_b, sE, inv_prn, kp3, ikp3, bze_b);
-
-    // ---- equilibrium state's normal slopes: a = slope_t(tunshift((We - WL) c)) for
-    // the u > 0 half, (WR - We) c for the u < 0 half, entering Be (cB, hp+1) and Fs (cF, hp+2)
-    {
-        const double ge = 0.5 * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
-        const double hp0 = 0.5 * g_erfc(-g_sqrt(E.lam) * E.u);
-        const double hm0 = 1.0 - hp0;
-        const double c = 2.0 / (E.den * dx);
-        for (int half = 0; half < 2; half++) {
-            double hh[7];
-            if (half == 0) half_tab<7>(sE.u, sE.th, hp0, E.u * hp0 + ge, hh);
-            else half_tab<7>(sE.u, sE.th, hm0, E.u * hm0 - ge, hh);
-            double a_b[5] = {0, 0, 0, 0, 0}, tmp[5] = {0, 0, 0, 0, 0}, bb[5];
-            mom_u_T(hh + 1, sE.th, kp2, kp4, Be_b, tmp);
-            for (int q = 0; q < 5; q++) a_b[q] = cBE * tmp[q];
-            for (int q = 0; q < 5; q++) tmp[q] = 0.0;
-            mom_u_T(hh + 2, sE.th, kp2, kp4, Fs_b, tmp);
-            for (int q = 0; q < 5; q++) a_b[q] += cFE * tmp[q];
-            slope_t_T(a_b, sE, inv_prn, kp3, ikp3, bb);
-            tunshift_T(bb, E.v, E.w);
-            double* out = half == 0 ? WLbar : WRbar;
-            const double sgn = half == 0 ? -c : c;
-            for (int q = 0; q < 5; q++) out[q] = sgn * bb[q];
-        }
-    }
-
-    // ---- the two sides (their sums entered bye, bze, Be, Fs after tshift to E's frame)
-    for (int side = 0; side < 2; side++) {
-        const Maxw<double>& S = side == 0 ? L : R;
-        const double* h = side == 0 ? hl : hr;
-        const TState<double> st{S.u, S.i2l, 2.0 * S.lam};
-        const double iden = 1.0 / S.den, dV = S.v - E.v, dW = S.w - E.w;
-        const double cB = (gm4 + gm5) * S.den, cA = gm5 * S.den, cF = p4 * S.den, cA2 = p5 * S.den;
-        double fu[6], hu[7];
-        full_tab<6>(st.u, st.th, fu);
-        half_tab<7>(st.u, st.th, h[0], h[1], hu);
-        double sy_b[5], sz_b[5], sB_b[5], sF_b[5];
-        for (int q = 0; q < 5; q++) {
-            sy_b[q] = bye_b[q];
-            sz_b[q] = bze_b[q];
-            sB_b[q] = Be_b[q];
-            sF_b[q] = Fs_b[q];
-        }
-        tshift_T(sy_b, dV, dW);
-        tshift_T(sz_b, dV, dW);
-        tshift_T(sB_b, dV, dW);
-        tshift_T(sF_b, dV, dW);
-        double ax_b[5] = {0, 0, 0, 0, 0}, ay_b[5] = {0, 0, 0, 0, 0}, az_b[5] = {0, 0, 0, 0, 0};
-        double tvw_b[5] = {0, 0, 0, 0, 0}, A_b[5] = {0, 0, 0, 0, 0}, tmp[5];
-        auto clear = [&] {
-            for (int q = 0; q < 5; q++) tmp[q] = 0.0;
-        };
-        // flux terms: sF = mom_u(hu+2, cF ax) + mom_uvw(hu+1, cF ay, 2) + mom_uvw(hu+1, cF az, 3)
-        //                  + mom_u(hu+1, cF tvw + cA2 A)
-        clear();
-        mom_u_T(hu + 2, st.th, kp2, kp4, sF_b, tmp);
-        for (int q = 0; q < 5; q++) ax_b[q] += cF * tmp[q];
-        clear();
-        mom_uvw_T(hu + 1, st.th, kp4, sF_b, tmp, 2);
-        for (int q = 0; q < 5; q++) ay_b[q] += cF * tmp[q];
-        clear();
-        mom_uvw_T(hu + 1, st.th, kp4, sF_b, tmp, 3);
-        for (int q = 0; q < 5; q++) az_b[q] += cF * tmp[q];
-        clear();
-        mom_u_T(hu + 1, st.th, kp2, kp4, sF_b, tmp);
-        for (int q = 0; q < 5; q++) {
-            tvw_b[q] += cF * tmp[q];
-            A_b[q] += cA2 * tmp[q];
-        }
-        // den_e B_ terms: sB = mom_u(hu+1, cB ax) + mom_uvw(hu, cB ay, 2) + mom_uvw(hu, cB az, 3)
-        //                      + mom_u(hu, cB tvw + cA A)
-        clear();
-        mom_u_T(hu + 1, st.th, kp2, kp4, sB_b, tmp);
-        for (int q = 0; q < 5; q++) ax_b[q] += cB * tmp[q];
-        clear();
-        mom_uvw_T(hu, st.th, kp4, sB_b, tmp, 2);
-        for (int q = 0; q < 5; q++) ay_b[q] += cB * tmp[q];
-        clear();
-        mom_uvw_T(hu, st.th, kp4, sB_b, tmp, 3);
-        for (int q = 0; q < 5; q++) az_b[q] += cB * tmp[q];
-        clear();
-        mom_u_T(hu, st.th, kp2, kp4, sB_b, tmp);
-        for (int q = 0; q < 5; q++) {
-            tvw_b[q] += cB * tmp[q];
-            A_b[q] += cA * tmp[q];
-        }
-        // equilibrium tangential right-hand sides: sy = mom_u(hu, ay), sz = mom_u(hu, az)
-        mom_u_T(hu, st.th, kp2, kp4, sy_b, ay_b);
-        mom_u_T(hu, st.th, kp2, kp4, sz_b, az_b);
-        // time slope: A = slope_t(-(mom_u(fu+1, ax) + mom_uvw(fu, ay, 2) + mom_uvw(fu, az, 3) + mom_u(fu, tvw)))
-        double nb[5];
-        slope_t_T(A_b, st, inv_prn, kp3, ikp3, nb);
-        for (int q = 0; q < 5; q++) nb[q] = -nb[q];
-        mom_u_T(fu + 1, st.th, kp2, kp4, nb, ax_b);
-        mom_uvw_T(fu, st.th, kp4, nb, ay_b, 2);
-        mom_uvw_T(fu, st.th, kp4, nb, az_b, 3);
-        mom_u_T(fu, st.th, kp2, kp4, nb, tvw_b);
-        for (int q = 0; q < 5; q++) {  // tvw = V ay + W az
-            ay_b[q] += S.v * tvw_b[q];
-            az_b[q] += S.w * tvw_b[q];
-        }
-        // micro slopes from the lab-frame macro slopes D iden (slope_from_lab: tunshift, then slope_t)
-        double* outs[3] = {Dbar[side], Dbar[2 + side], Dbar[4 + side]};
-        double* ins[3] = {ax_b, ay_b, az_b};
-        for (int k = 0; k < 3; k++) {
-            double b[5];
-            slope_t_T(ins[k], st, inv_prn, kp3, ikp3, b);
-            tunshift_T(b, S.v, S.w);
-            for (int q = 0; q < 5; q++) outs[k][q] = iden * b[q];
-        }
-    }
-}
-
 // flux of one face in the rotated frame from the 50 inputs x = (WL, WR, Wl, Wr,
 // D[6]) (each 5 values, rotated frame), collision time from WL
 template <typename T>
@@ -364,53 +150,6 @@ __host__ __device__ __forceinline__ void face_flux(const T x[50], T dt, T dx, co
 __host__ __device__ inline Gas<Dual> dual_gas(const Gas<double>& g)
 {
     return Gas<Dual>{Dual(g.gam), Dual(g.R), Dual(g.inv_prn), Dual(g.mu0), Dual(g.T0), Dual(g.eps), g.K};
-}
-
-// xbar[k] = sum_q Fbar[q] dF_q/dx_k for the 50 inputs of face_flux:
-//   slopes D and the cell states' normal-slope part: flux_affine_adjoint (one
-//   reverse pass);
-//   the collision time: one dual pass in tau, times dtau/dWL (five dual passes
-//   of collision_time);
-//   the WENO states Wl, Wr (in which the flux is not affine): one dual pass each.
-__host__ __device__ inline void face_flux_adjoint(const double x[50], double dt, double dx, const Gas<double>& g,
-                                                  const double Fbar[5], double xbar[50])
-{
-    const double tau = collision_time<double>(x, g);
-    {
-        double Dbar[6][5];
-        flux_affine_adjoint(x + 10, x + 15, tau, dt, dx, g, Fbar, xbar, xbar + 5, Dbar);
-        for (int d = 0; d < 6; d++)
-            for (int q = 0; q < 5; q++) xbar[20 + 5 * d + q] = Dbar[d][q];
-    }
-    const Gas<Dual> gd = dual_gas(g);
-    auto flux_dual = [&](int k, Dual taud, Dual F[5]) {
-        Dual xd[50];
-        for (int i = 0; i < 50; i++) xd[i] = Dual(x[i], i == k ? 1.0 : 0.0);
-        Dual D[6][5];
-        for (int d = 0; d < 6; d++)
-            for (int q = 0; q < 5; q++) D[d][q] = xd[20 + 5 * d + q];
-        gks_flux_tf<Dual, Dual>(xd, xd + 5, xd + 10, xd + 15, D, taud, Dual(dt), Dual(dx), gd, F);
-    };
-    {
-        Dual F[5];
-        flux_dual(-1, Dual(tau, 1.0), F);
-        double taub = 0.0;
-        for (int q = 0; q < 5; q++) taub += Fbar[q] * F[q].d;
-#pragma unroll 1
-        for (int k = 0; k < 5; k++) {
-            Dual w[5];
-            for (int i = 0; i < 5; i++) w[i] = Dual(x[i], i == k ? 1.0 : 0.0);
-            xbar[k] += taub * collision_time<Dual>(w, gd).d;
-        }
-    }
-#pragma unroll 1
-    for (int k = 10; k < 20; k++) {
-        Dual F[5];
-        flux_dual(k, Dual(tau), F);
-        double s = 0.0;
-        for (int q = 0; q < 5; q++) s += Fbar[q] * F[q].d;
-        xbar[k] = s;
-    }
 }
 
 }  // namespace gkm

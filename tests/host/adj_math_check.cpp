@@ -1,11 +1,12 @@
 // Reverse-mode pieces of the adjoint against forward-mode dual numbers:
 //  1. weno_side_grad on seeded stencils: smooth, kinked, discontinuous and
 //     constant-plus-noise data, both branches of the indicator maximum;
-//  2. face_flux_adjoint on seeded faces (air at 300 K, velocities up to Mach
-//     0.6, density and pressure jumps across the face, slopes of either sign):
-//     all 50 partials of face_flux (cell states, WENO states, six slopes; the
-//     collision time from the left cell) contracted with a random adjoint flux,
-//     against one dual pass per input.
+//  2. face_flux_vjp (one reverse pass through the whole face flux) on seeded
+//     faces (air at 300 K, velocities up to Mach 0.6, density and pressure
+//     jumps across the face, slopes of either sign): all 50 partials of
+//     face_flux (cell states, WENO states, six slopes; the collision time from
+//     the left cell) contracted with a random adjoint flux, against one dual
+//     pass per input.
 // Prints the largest relative differences; exit status 1 if either is above its
 // bound (argv[1] for the WENO part, argv[2] for the flux part).
 #include <cmath>
@@ -13,12 +14,12 @@
 #include <cstdlib>
 #include <random>
 
-#include "adj_math.hpp"
+#include "adj_flux.hpp"
 #include "gkm_dual.hpp"
 
 using gkm::Dual;
 
-static double check_face_flux_adjoint(int nfaces)
+static double check_face_flux_vjp(int nfaces)
 {
     std::mt19937_64 rng(777);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
@@ -57,7 +58,7 @@ static double check_face_flux_adjoint(int nfaces)
             for (int d = 0; d < 6; d++) x[20 + 5 * d + q] = D[d][q];
         }
         for (int q = 0; q < 5; q++) Fbar[q] = U(rng);
-        gkm::face_flux_adjoint(x, dt, dx, g, Fbar, xbar);
+        gkm::face_flux_vjp(x, dt, dx, g, Fbar, xbar);
         double xref[50], scale = 0.0;
         for (int k = 0; k < 50; k++) {
             Dual xd[50], F[5];
@@ -118,8 +119,8 @@ int main(int argc, char** argv)
                 "(smooth %.1e, kink %.1e, jump %.1e, nearly constant %.1e)\n",
                 n, worst, worst_kind[0], worst_kind[1], worst_kind[2], worst_kind[3]);
     const int nfaces = 4000;
-    const double fworst = check_face_flux_adjoint(nfaces);
-    std::printf("face_flux_adjoint vs dual numbers: %d faces x 50 inputs, largest relative difference %.3e\n",
+    const double fworst = check_face_flux_vjp(nfaces);
+    std::printf("face_flux_vjp vs dual numbers: %d faces x 50 inputs, largest relative difference %.3e\n",
                 nfaces, fworst);
     return worst <= bound && fworst <= fbound ? 0 : 1;
 }

@@ -1,5 +1,5 @@
-"""Reverse-mode pieces of the adjoint (csrc/adj_math.hpp) against forward-mode
-dual numbers (csrc/gkm_dual.hpp) on the host.
+"""Reverse-mode pieces of the adjoint (csrc/adj_math.hpp, csrc/adj_flux.hpp)
+against forward-mode dual numbers (csrc/gkm_dual.hpp) on the host.
 
 WENO gradient: 80000 seeded stencils (smooth, kinked, discontinuous, nearly
 constant).  Tolerance 1e-10 of the largest partial derivative of each stencil:
@@ -7,12 +7,16 @@ the two modes associate the same products differently, and the reverse mode
 forms q_k - out (candidate minus result), which cancels on nearly constant data
 (measured 5.6e-12 there, <= 2.9e-13 on the other kinds).
 
-Per-face flux adjoint: 4000 seeded faces (air at 300 K, up to Mach 0.6, weak
-and strong jumps), all 50 partials of the face flux (cell states, WENO states,
-six slopes; collision time from the left cell) contracted with a random adjoint
-flux, against one dual pass per input.  Tolerance 1e-12 of the largest partial
-times its input (measured 2.8e-15); dropping one term of the time-slope
-transpose reads 0.58, a wrong sign on the normal-slope part 2.0."""
+Per-face flux adjoint (face_flux_vjp, one reverse pass through the whole
+flux): 4000 seeded faces (air at 300 K, up to Mach 0.6, weak and strong
+jumps), all 50 partials of the face flux (cell states, WENO states, six slopes;
+collision time from the left cell) contracted with a random adjoint flux,
+against one dual pass per input.  Tolerance 1e-12 of the largest partial times
+its input (measured 2.8e-15).  Planted mistakes read far above it: the
+half-space moment's d/d lambda scaled by 0.8 gives 6.6e-2, the energy slot of
+c2p's adjoint dropped 1.0, the p3 coefficient's adjoint dropped 0.94, the
+pressure-jump term's d/d lambda dropped 0.61, one term of d f3 / d theta
+dropped 0.27."""
 import os
 import shutil
 import subprocess

@@ -38,6 +38,9 @@ METRIC = "GKM-WENO cell-updates/s at 512^3, FP64 vs mixed, % of HBM roofline"
 # is 80 B per cell-update (read W^n, write W^{n+1}).
 SWEEP_BYTES = {0: 80.0, 1: 120.0, 2: 120.0}
 STEP_BYTES = 80.0
+# the mixed path's compensated update also reads and writes an FP32 carry per
+# cell and variable in the z sweep (20 + 20 B)
+CARRY_BYTES = 40.0
 # Arithmetic operations of the reference's step with one WENO reconstruction
 # per interface (each +, -, *, / and transcendental counted once), counted
 # through the oracle restatement by oracle/count_ops.cpp: per interface (one
@@ -54,8 +57,6 @@ ALG_OPS_PER_CELL_UPDATE = 26951
 MEASURED_TRAFFIC_GIB = {
     ("mixed", 0, 2, 512): (11.24, 5.00),
     ("mixed", 1, 2, 512): (12.97, 5.00),
-    ("mixed", 2, 0, 512): (21.26, 5.00),
-    ("mixed", 2, 2, 512): (30.92, 5.20),
     ("fp64", 0, 2, 512): (58.84, 24.08),
     ("fp64", 1, 2, 512): (53.60, 21.58),
     ("fp64", 2, 0, 512): (60.67, 21.86),
@@ -216,6 +217,20 @@ def run_eulerian(gl, nc, P, W, prec_name, args):
     ms_per_step = 1e3 * el / args.steps
     value = nc ** 3 * args.steps / el
     log(f"bench: {prec_name}: {args.steps} steps in {el:.3f} s ({ms_per_step:.2f} ms/step)")
+    extra = {}
+    if prec == gl.GKM_MIXED and ctx.compensated:
+        # the same steps with the plain FP64 update (no carried rounding error)
+        ctx.set_compensation(False)
+        ctx.run(1)
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.run(args.steps)
+        ctx.sync()
+        el_plain = time.perf_counter() - t
+        ctx.set_compensation(True)
+        extra["no_compensation"] = {"ms_per_step": 1e3 * el_plain / args.steps,
+                                    "value": nc ** 3 * args.steps / el_plain}
+        log(f"bench: {prec_name}: without compensation {1e3 * el_plain / args.steps:.2f} ms/step")
     ctx.timing(True)
     ctx.run(args.profile_steps)
     ms, cnt = ctx.timing_read()
@@ -223,7 +238,7 @@ def run_eulerian(gl, nc, P, W, prec_name, args):
     kern = kernel_table(ms, cnt)
     log(f"bench: {prec_name}: kernel times {kern}")
     return ctx, {"value": value, "ms_per_step": ms_per_step, "sweep_tiles": list(tiles), "kernels": kern,
-                 "ms": ms, "cnt": cnt}
+                 "ms": ms, "cnt": cnt, **extra}
 
 
 def run_particles(ctx, npart, steps):
@@ -356,9 +371,21 @@ def run_adjoint(steps):
     pb.objective(u)
     s_fwd = time.perf_counter() - t
     ad = float(np.sum(g * d))
+    # central differences at the configuration's step h = 1e-6 |u0| (and 1e-8)
+    # along two band-limited and two white-noise unit directions (untimed)
+    dirs = {"band_limited": [adj.band_limited_direction(u.shape, 100 + k) for k in range(2)],
+            "white_noise": [adj.white_noise_direction(u.shape, 200 + k) for k in range(2)]}
+    fd = {}
+    for kind, ds in dirs.items():
+        _, _, rep = adj.check_gradient(pb, u, ds, (1e-6, 1e-8), g=g)
+        fd[kind] = {"err_fd": [r["err_fd"] for r in rep["rows"]], "err_tl": [r["err_tl"] for r in rep["rows"]]}
     out = {"grid": "64^3", "steps": steps, "checkpoint_every": 20, "precision": "fp64", "dt": pb.dt,
            "s_gradient": s_grad, "s_tangent_linear": s_tl, "s_forward": s_fwd,
-           "gradient_over_forward": s_grad / s_fwd, "adjoint_vs_tangent_linear": abs(ad - tl) / abs(tl)}
+           "gradient_over_forward": s_grad / s_fwd, "adjoint_vs_tangent_linear": abs(ad - tl) / abs(tl),
+           "finite_differences": fd,
+           "fd_note": "relative error of central differences of J against the adjoint gradient, h relative to |u0|; "
+                      "white-noise directions cross the step's branch switches (WENO maximum, |p_l - p_r|) "
+                      "in O(h) of the cells, so their error falls with h"}
     log(f"bench: adjoint {out}")
     return out
 
@@ -424,7 +451,8 @@ def main():
     ms, cnt = head["ms"], head["cnt"]
     dom = max(range(3), key=lambda i: ms[i] / max(cnt[i], 1))
     dom_ms = ms[dom] / cnt[dom]
-    achieved = SWEEP_BYTES[dom] * nc ** 3 / (dom_ms * 1e-3) / 1e9
+    launch_bytes = SWEEP_BYTES[dom] + (CARRY_BYTES if dom == 2 and "no_compensation" in head else 0.0)
+    achieved = launch_bytes * nc ** 3 / (dom_ms * 1e-3) / 1e9
     hbm80 = STEP_BYTES * nc ** 3 / (dom_ms * 1e-3) / 1e9
     alg_tops = ALG_OPS_PER_FACE * nc ** 3 / (dom_ms * 1e-3) / 1e12
     measured = MEASURED_TRAFFIC_GIB.get((precs[-1], dom, head["sweep_tiles"][dom], nc))
@@ -460,10 +488,10 @@ def main():
                                      "calibrated on a 1 GiB copy; profiles/sweep_traffic_512.md)"
                                      if traffic else "no counter measurement for this configuration",
                      "hbm": {"achieved_gbs": hbm80, "frac": hbm80 / HBM_PEAK_GBS, "bytes_per_cell": STEP_BYTES,
-                             "launch_alg_bytes_per_cell": SWEEP_BYTES[dom],
+                             "launch_alg_bytes_per_cell": launch_bytes,
                              "launch_alg_gbs": achieved, "launch_alg_frac": achieved / HBM_PEAK_GBS,
                              "traffic_over_launch_alg": None if traffic is None else
-                             traffic / (SWEEP_BYTES[dom] * nc ** 3),
+                             traffic / (launch_bytes * nc ** 3),
                              "peak_gbs": HBM_PEAK_GBS},
                      "valu_busy_pct_rocprof": MEASURED_VALU_BUSY.get((precs[-1], dom, head["sweep_tiles"][dom], nc)),
                      "valu_busy_note": "rocprofv3 VALUBusy of this kernel configuration "
@@ -475,6 +503,10 @@ def main():
         out["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
                        "sweep_tiles": res["fp64"]["sweep_tiles"], "kernels": res["fp64"]["kernels"]}
         out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    if "no_compensation" in head:
+        out["mixed_no_compensation"] = dict(head["no_compensation"], note=(
+            "the same mixed steps with the plain FP64 update instead of the compensated one "
+            "(gkm_set_compensation(ctx, 0))"))
     if particles is not None:
         out["particles"] = particles
     if args.ensemble_steps > 0:

@@ -18,7 +18,8 @@
 // The flux adjoint runs one workgroup per CU (launch bounds 256, 1): with 512
 // registers per lane and its side blocks in a function of their own it spills
 // least, and at 64^3 that measured fastest (0.60 s per 200-step gradient,
-// against 0.78 s inlined and 0.89 s at two workgroups per CU).
+// against 0.78 s inlined and 0.89 s at two workgroups per CU, with the WENO
+// adjoint at two workgroups per CU in all three).
 //
 // Face arrays hold one value per face and direction at the index of the face's
 // left cell.  Direction d: sweep axis n, tangential axes a, b as in the sweep
@@ -234,9 +235,11 @@ __global__ __launch_bounds__(256, 2) void adj_assemble_kernel(const double* __re
 // onto the other four slots and the reconstructed temperatures (closed-form
 // partial derivatives), then each one-dimensional reconstruction is
 // differentiated in reverse mode (weno_side_grad: all six cells in one pass),
-// and the temperature gradient reaches the cells through T(W).
+// and the temperature gradient reaches the cells through T(W).  One workgroup
+// per CU: with 512 registers per lane it spills 60 B per lane instead of
+// 1060 B (0.53 s per 64^3 200-step gradient against 0.60 s).
 template <int DIR>
-__global__ __launch_bounds__(256, 2) void weno_adj_kernel(const double* __restrict__ W, const double* __restrict__ WSbar,
+__global__ __launch_bounds__(256, 1) void weno_adj_kernel(const double* __restrict__ W, const double* __restrict__ WSbar,
                                                        long plane, int nc, Gas<double> g, double* __restrict__ Sbar)
 {
     const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -3,10 +3,15 @@
 //
 // The grid holds as many workgroups as the GPU keeps resident at once.  Each
 // phase of the step is a list of work items -- the workgroups the per-phase
-// launches would run -- and workgroup b takes the contiguous item range
-// [n b / G, n (b + 1) / G) of every list (b in XCD-grouped order, so each XCD
-// owns a contiguous range of members' pencils).  Phases are separated by a
-// grid barrier:
+// launches would run.  Items whose cost differs (sweep pencils of members of
+// different grid sizes; spline line groups) are handed out one at a time from
+// a counter per phase, the next to whichever workgroup is free, as the
+// hardware dispatcher does for the per-phase launches; a static split into
+// contiguous ranges gave the workgroups that drew the 64^3 members' pencils
+// twice the work of the others (sweep phases 1.9-2.1 ms against 1.1-1.3 ms
+// per launch).  Uniform items (CFL blocks, members' dt, tracer blocks) keep
+// the contiguous range [n b / G, n (b + 1) / G) of workgroup b.  Phases are
+// separated by a grid barrier:
 //   1  CFL block maxima of every member; tracer velocities at x^n
 //   2  dt of every member
 //   3  x sweeps   4  y sweeps   5  z sweeps (with the update)
@@ -75,10 +80,33 @@ __device__ __forceinline__ bool grid_barrier(const PersistStep& a, unsigned long
     return *failed != 0;
 }
 
-template <int DIR, bool MIX, int V>
-__device__ __forceinline__ void sweep_items_v(const PersistStep& a, int lo, int hi, double* lds)
+// The next item of phase `ph` for the whole workgroup, or -1 once its n items
+// are handed out.  Thread 0 takes it from the phase's counter; the workgroup
+// that is the last of the G to run out resets the counter and the tally for
+// the next launch (every other workgroup has by then made its last take).
+__device__ __forceinline__ int next_item(const PersistStep& a, int ph, int n, int* slot)
 {
-    for (int i = lo; i < hi; i++) {
+    if (threadIdx.x == 0) {
+        unsigned int* ctr = a.work + ph;
+        unsigned int* tally = a.work + kPersistPhases + ph;
+        const int i = (int)__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i >= n &&
+            __hip_atomic_fetch_add(tally, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(tally, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        *slot = i;
+    }
+    __syncthreads();
+    const int i = *slot;
+    __syncthreads();  // every thread has read the slot before thread 0 rewrites it
+    return i < n ? i : -1;
+}
+
+template <int DIR, bool MIX, int V>
+__device__ __forceinline__ void sweep_items_v(const PersistStep& a, int* slot, double* lds)
+{
+    for (int i; (i = next_item(a, DIR, a.nsweep[DIR], slot)) >= 0;) {
         const int m = entry_of(a.sfirst[DIR], a.nmem, i);
         const SweepArgs p = a.sweep[DIR][m];
         const int f0 = a.sfirst[DIR][m], nb = a.sfirst[DIR][m + 1] - f0;
@@ -88,12 +116,12 @@ __device__ __forceinline__ void sweep_items_v(const PersistStep& a, int lo, int 
 }
 
 template <int DIR, bool MIX>
-__device__ __forceinline__ void sweep_items(const PersistStep& a, int lo, int hi, double* lds)
+__device__ __forceinline__ void sweep_items(const PersistStep& a, int* slot, double* lds)
 {
     switch (a.tile[DIR]) {
-    case 0: sweep_items_v<DIR, MIX, 0>(a, lo, hi, lds); break;
-    case 1: sweep_items_v<DIR, MIX, 1>(a, lo, hi, lds); break;
-    default: sweep_items_v<DIR, MIX, 2>(a, lo, hi, lds); break;
+    case 0: sweep_items_v<DIR, MIX, 0>(a, slot, lds); break;
+    case 1: sweep_items_v<DIR, MIX, 1>(a, slot, lds); break;
+    default: sweep_items_v<DIR, MIX, 2>(a, slot, lds); break;
     }
 }
 
@@ -110,16 +138,14 @@ template <bool MIX>
 __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
 {
     __shared__ double lds[kSweepLdsMax];
-    __shared__ int failed;
+    __shared__ int failed, slot;
     // a launch after a timed-out one (same run, flag not yet cleared by the
     // host) does nothing; thread 0 reads the flag for the whole workgroup
     if (threadIdx.x == 0) failed = __hip_atomic_load(a.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (failed) return;
     const int G = gridDim.x;
-    // launch order: consecutive workgroups take consecutive slices of each
-    // phase's work and are dealt round-robin over the 8 XCDs, so every XCD gets
-    // the same mix of members (their slices differ in cost)
+    // uniform phases: consecutive workgroups take consecutive slices
     const int lb = blockIdx.x;
     auto range = [&](int n, int& lo, int& hi) {
         lo = (int)((long)n * lb / G);
@@ -160,19 +186,15 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
     }
     if (sync()) return;
     // 3-5: sweeps
-    range(a.nsweep[0], lo, hi);
-    sweep_items<0, MIX>(a, lo, hi, lds);
+    sweep_items<0, MIX>(a, &slot, lds);
     if (sync()) return;
-    range(a.nsweep[1], lo, hi);
-    sweep_items<1, MIX>(a, lo, hi, lds);
+    sweep_items<1, MIX>(a, &slot, lds);
     if (sync()) return;
-    range(a.nsweep[2], lo, hi);
-    sweep_items<2, MIX>(a, lo, hi, lds);
+    sweep_items<2, MIX>(a, &slot, lds);
     if (a.npart) {
         // 6: z pass of the splines of u, v, w of W^{n+1}
         if (sync()) return;
-        range(a.zblk, lo, hi);
-        for (int i = lo; i < hi; i++) {
+        for (int i; (i = next_item(a, 3, a.zblk, &slot)) >= 0;) {
             switch (a.zchunks) {
             case 1: zpass_item<1>(a, i); break;
             case 2: zpass_item<2>(a, i); break;
@@ -187,8 +209,7 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
             if (sync()) return;
             const int nsub = a.lblk[axis - 1];
             const int* first = a.lfirst + (axis - 1) * (3 * a.npart + 1);
-            range((nsub + 3) / 4, lo, hi);
-            for (int i = lo; i < hi; i++) {
+            for (int i; (i = next_item(a, 3 + axis, (nsub + 3) / 4, &slot)) >= 0;) {
                 const int sub = 4 * i + (tid >> 6);
                 if (sub >= nsub) continue;
                 const int e = lpt::ens_entry(first, 3 * a.npart, sub);

@@ -8,6 +8,13 @@
 
 namespace gkm {
 
+// phases of the persistent step that hand out their items dynamically
+constexpr int kPersistPhases = 6;
+// device words of the persistent step's synchronisation: barrier counter,
+// base, timeout flag (64-bit words), then the 2 kPersistPhases 32-bit work
+// counters and tallies
+constexpr int kPersistSyncWords = 3 + kPersistPhases;
+
 // Everything one step of every member reads, for one buffer parity.  The
 // tables are the ones the per-phase launches use (gkm_api.hip builds them).
 struct PersistStep {
@@ -35,6 +42,11 @@ struct PersistStep {
     unsigned long long* base;
     int* timeout;
     unsigned long long timeout_ticks;
+    // work counters of the phases whose items differ in cost (x, y, z sweeps;
+    // z, x, y passes of the splines) and, from work + kPersistPhases, the
+    // number of workgroups that have run out of each phase's items; both zero
+    // between launches
+    unsigned int* work;
     // optional: workgroup 0 writes the wall clock at the start and after every
     // barrier (10 entries), for per-phase timings
     unsigned long long* trace;

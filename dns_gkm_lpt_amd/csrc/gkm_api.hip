@@ -1013,7 +1013,7 @@ struct gkm_ensemble {
     // persistent mode: the whole step is one launch of ens_step_kernel
     bool persistent = false;
     int pgrid = 0;                     // resident workgroups
-    unsigned long long* dbar = nullptr;  // barrier counter, base, timeout flag
+    unsigned long long* dbar = nullptr;  // barrier counter, base, timeout flag, work counters
     unsigned long long timeout_ticks = 0;
     std::string err;
 };
@@ -1134,6 +1134,7 @@ gkm::PersistStep persist_args(const gkm_ensemble* e, int p)
     a.base = e->dbar + 1;
     a.timeout = reinterpret_cast<int*>(e->dbar + 2);
     a.timeout_ticks = e->timeout_ticks;
+    a.work = reinterpret_cast<unsigned int*>(e->dbar + 3);
     return a;
 }
 
@@ -1459,9 +1460,9 @@ int gkm_ensemble_run(gkm_ensemble* e, int nsteps, int use_graph)
         int timed_out = 0;
         EHIPCHK(e, hipMemcpy(&timed_out, e->dbar + 2, sizeof(int), hipMemcpyDeviceToHost));
         if (timed_out) {
-            // the counter and base of the failed launches are stale: start the
-            // barrier afresh (graph replays read the same device words)
-            EHIPCHK(e, hipMemset(e->dbar, 0, 3 * sizeof(unsigned long long)));
+            // the counter, base and work counters of the failed launches are
+            // stale: start them afresh (graph replays read the same device words)
+            EHIPCHK(e, hipMemset(e->dbar, 0, gkm::kPersistSyncWords * sizeof(unsigned long long)));
             return efail(e, GKM_EHIP, "persistent step: a grid barrier timed out (workgroups not all resident)");
         }
     }
@@ -1496,9 +1497,9 @@ int gkm_ensemble_set_persistent(gkm_ensemble* e, int on)
         EHIPCHK(e, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         if (khz <= 0) return efail(e, GKM_EHIP, "persistent step: no wall-clock rate");
         e->timeout_ticks = 2ull * 1000ull * (unsigned long long)khz;  // 2 s per barrier wait
-        if (hipMalloc(&e->dbar, 3 * sizeof(unsigned long long)) != hipSuccess)
+        if (hipMalloc(&e->dbar, gkm::kPersistSyncWords * sizeof(unsigned long long)) != hipSuccess)
             return efail(e, GKM_ENOMEM, "hipMalloc barrier");
-        EHIPCHK(e, hipMemset(e->dbar, 0, 3 * sizeof(unsigned long long)));
+        EHIPCHK(e, hipMemset(e->dbar, 0, gkm::kPersistSyncWords * sizeof(unsigned long long)));
     }
     if (e->persistent != (on != 0)) drop_graphs(e);
     e->persistent = on != 0;

@@ -11,10 +11,10 @@
 
 namespace gkm {
 
-template <int DIR, bool MIX, int V>
+template <int DIR, bool MIX, int V, bool GEN, int CMP>
 __global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
 {
-    sweep_body<DIR, MIX, V>(p, blockIdx.x, gridDim.x, true);
+    sweep_body<DIR, MIX, V, false, GEN, CMP>(p, blockIdx.x, gridDim.x, true);
 }
 
 // One launch for the sweeps of every member of an ensemble: member m owns the
@@ -110,10 +110,25 @@ static hipError_t launch_tile(const SweepArgs& a, bool mixed, hipStream_t s)
     if (!sweep_tile_ok(DIR, V, a.nc) || a.seg <= 0 || a.seg % CN || a.nseg != cdiv(a.nc, a.seg))
         return hipErrorInvalidValue;
     const long nblk = (long)cdiv(a.nc, CA) * cdiv(a.nc, CB) * a.nseg;
-    if (mixed)
-        hipLaunchKernelGGL((sweep_kernel<DIR, true, V>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
-    else
-        hipLaunchKernelGGL((sweep_kernel<DIR, false, V>), dim3((unsigned)nblk), dim3(CN * CA * CB), 0, s, a);
+    // the compensated update is the mixed z sweep's (an FP64 context has no carry)
+    if ((a.cmp_out != nullptr) != (a.cmp_in != nullptr) || (a.cmp_out && !mixed)) return hipErrorInvalidValue;
+    const bool gen = a.nc % CA || a.nc % CB || a.nc % a.seg;  // some workgroup reaches past the grid
+    const bool cmp = a.cmp_out && a.mode == 2;
+    const dim3 grid((unsigned)nblk), block(CN * CA * CB);
+    if (mixed) {
+        if constexpr (DIR == 2) {
+            if (cmp) {
+                if (gen) hipLaunchKernelGGL((sweep_kernel<DIR, true, V, true, 1>), grid, block, 0, s, a);
+                else hipLaunchKernelGGL((sweep_kernel<DIR, true, V, false, 1>), grid, block, 0, s, a);
+                return hipGetLastError();
+            }
+        }
+        if (gen) hipLaunchKernelGGL((sweep_kernel<DIR, true, V, true, 0>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((sweep_kernel<DIR, true, V, false, 0>), grid, block, 0, s, a);
+    } else {
+        if (gen) hipLaunchKernelGGL((sweep_kernel<DIR, false, V, true, 0>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((sweep_kernel<DIR, false, V, false, 0>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -130,7 +130,13 @@ constexpr int kSweepLdsMax = sweep_lds_max<0>() > sweep_lds_max<1>()
 // EXT = false: the body's own LDS arrays; EXT = true: the caller's buffer lds
 // of at least sweep_lds_doubles<DIR, V>() doubles (a kernel running several
 // bodies shares one buffer between them).
-template <int DIR, bool MIX, int V, bool EXT = false>
+// GEN = true: any grid (tiles and segments may reach past nc); GEN = false:
+// only for grids the tile and the segment divide (no line or face of the
+// workgroup lies past the grid, so none is masked or wrapped into range: the
+// general form's extra live values made the mixed x and y sweeps spill).
+// CMP (the z sweep's update): -1 compensated when p.cmp_out is set, 0 plain,
+// 1 compensated (the fixed forms leave the other update out of the kernel).
+template <int DIR, bool MIX, int V, bool EXT = false, bool GEN = true, int CMP = -1>
 __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk, bool remap, double* lds = nullptr)
 {
     constexpr int CN = Tile<DIR, V>::CN, CA = Tile<DIR, V>::CA, CB = Tile<DIR, V>::CB;
@@ -157,7 +163,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
     if (remap && (nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
     const int s = bid % p.nseg;
     const int pencil = bid / p.nseg;
-    const int npb = (nc + CB - 1) / CB;  // pencils across b; the last ones may reach past nc
+    const int npb = GEN ? (nc + CB - 1) / CB : nc / CB;  // pencils across b; with GEN the last may reach past nc
     const int a0 = (pencil / npb) * CA;
     const int b0 = (pencil % npb) * CB;
 
@@ -175,7 +181,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
     const int lc = ta * CB + tb;
     // a grid that is not a multiple of the tile: lines past nc (their cells
     // wrap) and faces past the segment's end are computed, not written
-    const bool live = a0 + ta < nc && b0 + tb < nc;
+    const bool live = !GEN || (a0 + ta < nc && b0 + tb < nc);
 
     const double dt = *p.dt;
     const double dx = p.dx;
@@ -231,7 +237,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
         // ---- one GKS flux per thread
         {
             const int fn = n0 + tn;
-            const int ca = wrapi(a0 + ta, nc), cb = wrapi(b0 + tb, nc);
+            const int ca = GEN ? wrapi(a0 + ta, nc) : a0 + ta, cb = GEN ? wrapi(b0 + tb, nc) : b0 + tb;
             in.iL = cell_index<DIR>(wrapi(fn, nc), ca, cb, nc);
             in.iR = cell_index<DIR>(wrapi(fn + 1, nc), ca, cb, nc);
             double tau;
@@ -252,15 +258,18 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
         __syncthreads();
 
         // ---- flux differences of the chunk's cells
-        if (c >= 0 && live && n0 + tn < seg_end) {
+        if (c >= 0 && live && (!GEN || n0 + tn < seg_end)) {
             const long idx = cell_index<DIR>(n0 + tn, a0 + ta, b0 + tb, nc);
 #pragma unroll
             for (int q = 0; q < 5; q++) {
                 const double dF = sF[sf_at<DIR, CN, CA * CB>(q, tn + 1, lc)] - sF[sf_at<DIR, CN, CA * CB>(q, tn, lc)];
                 const long o = q * plane + idx;
-                if (p.mode == 0) p.dW[o] = dF;
-                else if (p.mode == 1) p.dW[o] += dF;
-                else if (p.cmp_out) {
+                // the x sweep only ever starts the sum; the y sweep adds to it
+                // unless asked for its flux differences alone (mode 0), the z
+                // sweep updates unless asked for its flux differences alone
+                if (DIR == 0 || p.mode == 0) p.dW[o] = dF;
+                else if (DIR == 1) p.dW[o] += dF;
+                else if (CMP == 1 || (CMP < 0 && p.cmp_out)) {
                     // W^{n+1} + e = W^n + (increment + carried error), exactly (TwoSum)
                     const double w = W[o];
                     const double y = (double)p.cmp_in[o] - p.inv_dx * (p.dW[o] + dF);

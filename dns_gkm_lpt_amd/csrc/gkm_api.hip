@@ -993,6 +993,8 @@ struct gkm_ensemble {
     gkm::SweepArgs* dargs = nullptr;  // [parity][direction][member]
     int* dfirst = nullptr;            // [direction][member + 1] first block of each member
     int nblk[3] = {0, 0, 0};
+    bool exact[3] = {false, false, false};  // per direction: tile and segment divide every member's grid
+    bool compensated = false;               // every member's z sweep takes the compensated update
     gkm::CflArgs* dcfl = nullptr;     // [parity][member]
     double* partial = nullptr;        // n * kEnsCflBlocks
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1070,17 +1072,27 @@ int ens_build(gkm_ensemble* e, const int t[3])
     const int n = e->n;
     std::vector<gkm::SweepArgs> args((size_t)6 * n);
     std::vector<int> first((size_t)3 * (n + 1));
+    // the sweep kernels take the update's kind as a compile-time choice: the
+    // members must agree (mixed members compensate unless switched off alone,
+    // which members cannot be)
+    int ncomp = 0;
+    for (gkm_ctx* c : e->mem) ncomp += c->compensate ? 1 : 0;
+    if (ncomp != 0 && ncomp != n) return efail(e, GKM_EINVAL, "ensemble members differ in compensation");
     for (int d = 0; d < 3; d++) {
         int acc = 0;
+        bool exact = true;
         for (int m = 0; m < n; m++) {
             first[(size_t)d * (n + 1) + m] = acc;
             for (int p = 0; p < 2; p++) args[((size_t)p * 3 + d) * n + m] = ens_sweep_args(e->mem[m], p, d, t[d]);
             const gkm::SweepArgs& a = args[(size_t)d * n + m];
             acc += gkm::sweep_blocks(d, a.nc, t[d], a.seg);
+            exact = exact && gkm::sweep_exact(d, t[d], a.nc, a.seg);
         }
         first[(size_t)d * (n + 1) + n] = acc;
         e->nblk[d] = acc;
+        e->exact[d] = exact;
     }
+    e->compensated = ncomp == n;
     std::vector<gkm::CflArgs> cfl((size_t)2 * n);
     for (int p = 0; p < 2; p++)
         for (int m = 0; m < n; m++) {
@@ -1150,7 +1162,7 @@ hipError_t ens_step(gkm_ensemble* e, int p, bool capturing)
     if (r == hipSuccess) r = gkm::launch_ens_cfl(e->dcfl + (size_t)p * n, n, e->partial, e->stream);
     for (int d = 0; d < 3 && r == hipSuccess; d++)
         r = gkm::launch_ens_sweep(d, e->tile[d], e->dargs + ((size_t)p * 3 + d) * n, e->dfirst + (size_t)d * (n + 1),
-                                  n, e->nblk[d], e->mixed, e->stream);
+                                  n, e->nblk[d], e->mixed, e->exact[d], e->compensated, e->stream);
     if (np && r == hipSuccess)
         r = lpt::launch_ens_prefilter(e->dz + (size_t)p * np, e->dzfirst, np, e->zblk, e->zchunks,
                                       e->dline + (size_t)p * 3 * np, e->dlfirst, 3 * np, e->lblk, e->stream);
@@ -1393,7 +1405,7 @@ int gkm_ensemble_autotune(gkm_ensemble* e, int reps, int tile[3], double ms[9])
             // dW is scratch and Wout the next state's buffer: the members' states are not changed
             auto launch = [&] {
                 return gkm::launch_ens_sweep(d, v, e->dargs + ((size_t)p * 3 + d) * n, e->dfirst + (size_t)d * (n + 1),
-                                             n, e->nblk[d], e->mixed, e->stream);
+                                             n, e->nblk[d], e->mixed, e->exact[d], e->compensated, e->stream);
             };
             EHIPCHK(e, launch());
             EHIPCHK(e, hipEventRecord(e->ev[0], e->stream));

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(SweepArgs p)
 // contiguous ranges per XCD would hand one XCD the long ones; dealt round-robin,
 // every XCD gets the same mix (64 members at 32^3-64^3: 5.34 against 5.90 ms
 // per step).
-template <int DIR, bool MIX, int V>
+template <int DIR, bool MIX, int V, bool GEN, int CMP>
 __global__ __launch_bounds__(256, 2) void ens_sweep_kernel(const SweepArgs* __restrict__ members,
                                                           const int* __restrict__ first, int nmem)
 {
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256, 2) void ens_sweep_kernel(const SweepArgs* __re
         else hi = mid;
     }
     const SweepArgs p = members[lo];
-    sweep_body<DIR, MIX, V>(p, g - first[lo], first[lo + 1] - first[lo], false);
+    sweep_body<DIR, MIX, V, false, GEN, CMP>(p, g - first[lo], first[lo + 1] - first[lo], false);
 }
 
 static int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -112,7 +112,7 @@ static hipError_t launch_tile(const SweepArgs& a, bool mixed, hipStream_t s)
     const long nblk = (long)cdiv(a.nc, CA) * cdiv(a.nc, CB) * a.nseg;
     // the compensated update is the mixed z sweep's (an FP64 context has no carry)
     if ((a.cmp_out != nullptr) != (a.cmp_in != nullptr) || (a.cmp_out && !mixed)) return hipErrorInvalidValue;
-    const bool gen = a.nc % CA || a.nc % CB || a.nc % a.seg;  // some workgroup reaches past the grid
+    const bool gen = !sweep_exact(DIR, V, a.nc, a.seg);  // some workgroup reaches past the grid
     const bool cmp = a.cmp_out && a.mode == 2;
     const dim3 grid((unsigned)nblk), block(CN * CA * CB);
     if (mixed) {
@@ -159,36 +159,53 @@ int sweep_blocks(int dir, int nc, int tile, int seg)
     return cdiv(nc, CA) * cdiv(nc, CB) * cdiv(nc, seg);
 }
 
+bool sweep_exact(int dir, int tile, int nc, int seg)
+{
+    if (!sweep_tile_ok(dir, tile, nc) || seg <= 0) return false;
+    return nc % kTiles[dir][tile][1] == 0 && nc % kTiles[dir][tile][2] == 0 && nc % seg == 0;
+}
+
 template <int DIR, int V>
 static hipError_t launch_ens_tile(const SweepArgs* members, const int* first, int nmem, int nblk, bool mixed,
-                                  hipStream_t s)
+                                  bool exact, bool cmp, hipStream_t s)
 {
-    if (mixed)
-        hipLaunchKernelGGL((ens_sweep_kernel<DIR, true, V>), dim3((unsigned)nblk), dim3(256), 0, s, members, first, nmem);
-    else
-        hipLaunchKernelGGL((ens_sweep_kernel<DIR, false, V>), dim3((unsigned)nblk), dim3(256), 0, s, members, first, nmem);
+    const dim3 grid((unsigned)nblk), block(256);
+    if (mixed) {
+        if constexpr (DIR == 2) {
+            if (cmp) {
+                if (exact) hipLaunchKernelGGL((ens_sweep_kernel<DIR, true, V, false, 1>), grid, block, 0, s, members, first, nmem);
+                else hipLaunchKernelGGL((ens_sweep_kernel<DIR, true, V, true, 1>), grid, block, 0, s, members, first, nmem);
+                return hipGetLastError();
+            }
+        }
+        if (exact) hipLaunchKernelGGL((ens_sweep_kernel<DIR, true, V, false, 0>), grid, block, 0, s, members, first, nmem);
+        else hipLaunchKernelGGL((ens_sweep_kernel<DIR, true, V, true, 0>), grid, block, 0, s, members, first, nmem);
+    } else {
+        if (exact) hipLaunchKernelGGL((ens_sweep_kernel<DIR, false, V, false, 0>), grid, block, 0, s, members, first, nmem);
+        else hipLaunchKernelGGL((ens_sweep_kernel<DIR, false, V, true, 0>), grid, block, 0, s, members, first, nmem);
+    }
     return hipGetLastError();
 }
 
 template <int DIR>
 static hipError_t launch_ens_dir(int tile, const SweepArgs* members, const int* first, int nmem, int nblk, bool mixed,
-                                 hipStream_t s)
+                                 bool exact, bool cmp, hipStream_t s)
 {
     switch (tile) {
-    case 0: return launch_ens_tile<DIR, 0>(members, first, nmem, nblk, mixed, s);
-    case 1: return launch_ens_tile<DIR, 1>(members, first, nmem, nblk, mixed, s);
-    case 2: return launch_ens_tile<DIR, 2>(members, first, nmem, nblk, mixed, s);
+    case 0: return launch_ens_tile<DIR, 0>(members, first, nmem, nblk, mixed, exact, cmp, s);
+    case 1: return launch_ens_tile<DIR, 1>(members, first, nmem, nblk, mixed, exact, cmp, s);
+    case 2: return launch_ens_tile<DIR, 2>(members, first, nmem, nblk, mixed, exact, cmp, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_ens_sweep(int dir, int tile, const SweepArgs* members, const int* first, int nmem, int nblk,
-                            bool mixed, hipStream_t s)
+                            bool mixed, bool exact, bool compensated, hipStream_t s)
 {
-    if (nmem <= 0 || nblk <= 0) return hipErrorInvalidValue;
-    if (dir == 0) return launch_ens_dir<0>(tile, members, first, nmem, nblk, mixed, s);
-    if (dir == 1) return launch_ens_dir<1>(tile, members, first, nmem, nblk, mixed, s);
-    if (dir == 2) return launch_ens_dir<2>(tile, members, first, nmem, nblk, mixed, s);
+    if (nmem <= 0 || nblk <= 0 || (compensated && !mixed)) return hipErrorInvalidValue;
+    if (dir == 0) return launch_ens_dir<0>(tile, members, first, nmem, nblk, mixed, exact, compensated, s);
+    if (dir == 1) return launch_ens_dir<1>(tile, members, first, nmem, nblk, mixed, exact, compensated, s);
+    if (dir == 2) return launch_ens_dir<2>(tile, members, first, nmem, nblk, mixed, exact, compensated, s);
     return hipErrorInvalidValue;
 }
 

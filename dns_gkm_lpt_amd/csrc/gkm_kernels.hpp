@@ -49,9 +49,14 @@ hipError_t launch_sweep(int dir, const SweepArgs& a, bool mixed, hipStream_t s);
 // blocks of one grid's sweep launch for a tile variant and segment length
 int sweep_blocks(int dir, int nc, int tile, int seg);
 // one launch sweeping every member of an ensemble: members[m] (device array) with
-// blocks [first[m], first[m+1]) of nblk = first[nmem]; one tile variant for all
+// blocks [first[m], first[m+1]) of nblk = first[nmem]; one tile variant for all.
+// exact: the tile and segment divide every member's grid (sweep_exact);
+// compensated: every member's z sweep takes the compensated update (mixed
+// members), else none does.
 hipError_t launch_ens_sweep(int dir, int tile, const SweepArgs* members, const int* first, int nmem, int nblk,
-                            bool mixed, hipStream_t s);
+                            bool mixed, bool exact, bool compensated, hipStream_t s);
+// the tile variant and the segment length divide nc: no workgroup reaches past the grid
+bool sweep_exact(int dir, int tile, int nc, int seg);
 // faces per workgroup chosen for a grid: slots = concurrent workgroups of the
 // device for a single grid, 0 for the members of an ensemble
 int sweep_seg_len(int dir, int nc, int tile, int slots);

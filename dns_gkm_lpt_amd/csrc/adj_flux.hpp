@@ -441,7 +441,6 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
                                               const double Fbar[5], double xbar[50])
 {
     using namespace vjp;
-    const double pi = 3.141592653589793238462643383279502884;
     const FluxK k{double(g.K + 2), double(g.K + 3), double(g.K + 4), 1.0 / double(g.K + 3), g.inv_prn};
     const double* WL = x;
     const double* WR = x + 5;
@@ -459,13 +458,17 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
     const TimeCoefs<Dual> tcd = time_coefs<Dual>(Dual(taue, 1.0), Dual(dt));
     const TimeCoefs<double> tc{tcd.gm1.v, tcd.gm2.v, tcd.gm3.v, tcd.gm4.v, tcd.gm5.v, tcd.p0.v,
                                tcd.p1.v,  tcd.p2.v,  tcd.p3.v,  tcd.p4.v,  tcd.p5.v};
+    // the half-space starts as gks_flux_tf forms them
+    const double isqpi = 0.5641895835477562869480794515607726;  // 1/sqrt(pi)
     double hl[4], hr[4];
     {
-        const double gl = 0.5 * g_exp(-L.lam * L.u * L.u) / g_sqrt(L.lam * pi);
-        const double h0 = 0.5 * g_erfc(-g_sqrt(L.lam) * L.u);
+        const double sl = g_sqrt(L.lam), zl = sl * L.u, el = g_exp(-zl * zl);
+        const double gl = 0.5 * isqpi * el / sl;
+        const double h0 = half_erfc(-zl, el);
         half_tab<4>(L.u, L.i2l, h0, L.u * h0 + gl, hl);
-        const double gr = 0.5 * g_exp(-R.lam * R.u * R.u) / g_sqrt(R.lam * pi);
-        const double r0 = 0.5 * g_erfc(g_sqrt(R.lam) * R.u);
+        const double sr = g_sqrt(R.lam), zr = sr * R.u, er = g_exp(-zr * zr);
+        const double gr = 0.5 * isqpi * er / sr;
+        const double r0 = half_erfc(zr, er);
         half_tab<4>(R.u, R.i2l, r0, R.u * r0 - gr, hr);
     }
     // the half Maxwellians' <psi> (equilibrium) and <u psi> (p3 flux) before the
@@ -519,8 +522,9 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
         }
     }
     // equilibrium halves with the cells' normal slopes
-    const double ge = 0.5 * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
-    const double hp0 = 0.5 * g_erfc(-g_sqrt(E.lam) * E.u);
+    const double se = g_sqrt(E.lam), ze = se * E.u, ee = g_exp(-ze * ze);
+    const double ge = 0.5 * isqpi * ee / se;
+    const double hp0 = half_erfc(-ze, ee);
     const double hm0 = 1.0 - hp0;
     const double cn = 2.0 / (E.den * dx);
     const double cBE = tc.gm2 * E.den, cFE = tc.p2 * E.den;

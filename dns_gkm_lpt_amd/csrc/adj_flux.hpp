@@ -453,7 +453,7 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
     Maxw<double> L, R;
     maxw_init(L, Wl, g);
     maxw_init(R, Wr, g);
-    const double pl = L.den / L.lam, pr = R.den / R.lam;
+    const double pl = 2.0 * L.den * L.i2l, pr = 2.0 * R.den * R.i2l;  // rho / lambda, as the flux forms it
     const double taue = tau0 + dt * (::fabs(pl - pr) / ::fabs(pl + pr));
     const TimeCoefs<Dual> tcd = time_coefs<Dual>(Dual(taue, 1.0), Dual(dt));
     const TimeCoefs<double> tc{tcd.gm1.v, tcd.gm2.v, tcd.gm3.v, tcd.gm4.v, tcd.gm5.v, tcd.p0.v,

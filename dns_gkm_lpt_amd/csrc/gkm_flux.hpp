@@ -144,7 +144,7 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         maxw_init(R, Wr, g);
     }
     {   // pressure-jump term of the collision time (GKMflux.cpp:156-157)
-        const TE pl = L.den / L.lam, pr = R.den / R.lam;
+        const TE pl = TE(2) * L.den * L.i2l, pr = TE(2) * R.den * R.i2l;  // rho / lambda
         tau = tau + dt * (fabs(pl - pr) / fabs(pl + pr));
     }
     // time-integration coefficients (GKMflux.cpp:162-167, 197-204)

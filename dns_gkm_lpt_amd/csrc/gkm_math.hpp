@@ -88,11 +88,14 @@ template <>
 __host__ __device__ __forceinline__ double half_erfc<double>(double x, double E)
 {
     const double t = fabs(x);
-    const double y = (t - 3.0) / (t + 3.0);
+    // one division for both y = (t - 3)/(t + 3) and 1/(1 + 2t)
+    const double a = t + 3.0, b = 1.0 + 2.0 * t;
+    const double r = 1.0 / (a * b);
+    const double y = (t - 3.0) * b * r;
     double acc = kErfcxP[24];
 #pragma unroll
     for (int i = 23; i >= 0; i--) acc = acc * y + kErfcxP[i];
-    const double v = 0.5 * E * (2.0 * acc / (1.0 + 2.0 * t));  // 0.5 erfc(t)
+    const double v = E * acc * a * r;  // 0.5 erfc(t) = 0.5 E 2/(1 + 2t) P(y)
     return x >= 0.0 ? v : 1.0 - v;
 }
 

@@ -51,11 +51,13 @@ def half_erfc(x, p):
     """0.5 erfc(x) in double precision, the device code's operation order."""
     t = abs(x)
     E = np.exp(-x * x)
-    y = (t - K) / (t + K)
+    a, b = t + K, 1.0 + 2.0 * t
+    r = 1.0 / (a * b)
+    y = (t - K) * b * r
     acc = p[-1]
-    for a in p[-2::-1]:
-        acc = acc * y + a
-    v = 0.5 * E * (2.0 * acc / (1.0 + 2.0 * t))
+    for c in p[-2::-1]:
+        acc = acc * y + c
+    v = E * acc * a * r
     return v if x >= 0 else 1.0 - v
 
 

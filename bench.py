@@ -57,34 +57,34 @@ ALG_OPS_PER_CELL_UPDATE = 26951
 # Mixed rows: the compensated update (the default); x and y sweeps do not
 # touch the carry.
 MEASURED_TRAFFIC_GIB = {
-    ("mixed", 0, 0, 512): (7.81, 5.00),
-    ("mixed", 0, 2, 512): (10.90, 5.00),
-    ("mixed", 1, 0, 512): (13.88, 5.00),
-    ("mixed", 1, 2, 512): (13.05, 5.00),
-    ("mixed", 2, 0, 512): (29.16, 7.71),
-    ("mixed", 2, 2, 512): (51.62, 7.60),
-    ("fp64", 0, 0, 512): (47.75, 24.23),
-    ("fp64", 0, 2, 512): (58.19, 24.10),
-    ("fp64", 1, 0, 512): (61.88, 23.73),
-    ("fp64", 1, 2, 512): (57.10, 21.58),
-    ("fp64", 2, 0, 512): (60.43, 21.86),
-    ("fp64", 2, 2, 512): (74.40, 21.43),
+    ("mixed", 0, 0, 512): (7.83, 5.00),
+    ("mixed", 0, 2, 512): (8.38, 5.00),
+    ("mixed", 1, 0, 512): (13.86, 5.00),
+    ("mixed", 1, 2, 512): (13.00, 5.00),
+    ("mixed", 2, 0, 512): (28.43, 7.67),
+    ("mixed", 2, 2, 512): (51.76, 7.59),
+    ("fp64", 0, 0, 512): (48.15, 24.23),
+    ("fp64", 0, 2, 512): (59.87, 24.09),
+    ("fp64", 1, 0, 512): (58.55, 21.70),
+    ("fp64", 1, 2, 512): (59.33, 21.58),
+    ("fp64", 2, 0, 512): (57.53, 20.82),
+    ("fp64", 2, 2, 512): (79.18, 21.48),
 }
 # rocprofv3 VALUBusy (% of time the VALU issues) of each sweep launch, keyed
 # like MEASURED_TRAFFIC_GIB; commands and table in profiles/sweep_valu_512.md.
 MEASURED_VALU_BUSY = {
-    ("mixed", 0, 0, 512): 83.6,
-    ("mixed", 0, 2, 512): 80.6,
-    ("mixed", 1, 0, 512): 73.0,
-    ("mixed", 1, 2, 512): 73.2,
-    ("mixed", 2, 0, 512): 76.7,
-    ("mixed", 2, 2, 512): 69.3,
-    ("fp64", 0, 0, 512): 57.0,
-    ("fp64", 0, 2, 512): 54.7,
-    ("fp64", 1, 0, 512): 49.6,
-    ("fp64", 1, 2, 512): 50.6,
-    ("fp64", 2, 0, 512): 53.5,
-    ("fp64", 2, 2, 512): 51.8,
+    ("mixed", 0, 0, 512): 83.1,
+    ("mixed", 0, 2, 512): 79.6,
+    ("mixed", 1, 0, 512): 71.5,
+    ("mixed", 1, 2, 512): 71.7,
+    ("mixed", 2, 0, 512): 73.8,
+    ("mixed", 2, 2, 512): 65.5,
+    ("fp64", 0, 0, 512): 55.5,
+    ("fp64", 0, 2, 512): 53.5,
+    ("fp64", 1, 0, 512): 50.3,
+    ("fp64", 1, 2, 512): 48.4,
+    ("fp64", 2, 0, 512): 53.4,
+    ("fp64", 2, 2, 512): 48.7,
 }
 KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
                 3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",

@@ -18,6 +18,14 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+# Per-source flags.  The sweep kernels keep their uniform values (kernel
+# arguments, per-variable base addresses, FP64 constants) in scalar registers;
+# with machine-level loop-invariant code motion on, hoisting the constants'
+# materialisation out of the chunk loop overflowed the 102 SGPRs, and the
+# spilled SGPRs were reloaded inside the loop with v_readlane (a VALU
+# instruction each, ~100 per face in the compensated z sweep).  Without it the
+# constants are rematerialised in the loop by scalar moves.
+SOURCE_FLAGS = {"gkm_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
 def build(resource_usage=False, verbose=False, jobs=None):
@@ -25,7 +33,7 @@ def build(resource_usage=False, verbose=False, jobs=None):
     objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + FLAGS + SOURCE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if resource_usage:
             cmd.append("-Rpass-analysis=kernel-resource-usage")
         cmds.append(cmd)

@@ -145,12 +145,12 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
     }
     {   // pressure-jump term of the collision time (GKMflux.cpp:156-157)
         const TE pl = TE(2) * L.den * L.i2l, pr = TE(2) * R.den * R.i2l;  // rho / lambda
-        tau = tau + dt * (fabs(pl - pr) / fabs(pl + pr));
+        tau = tau + dt * g_div(fabs(pl - pr), fabs(pl + pr));
     }
     // time-integration coefficients (GKMflux.cpp:162-167, 197-204)
-    const TE ex = g_exp(-dt / tau);
+    const TE ex = g_exp(-g_div(dt, tau));
     const TE om = TE(1) - ex;
-    const TE igm0 = TE(1) / (dt - tau * om);
+    const TE igm0 = g_rcp(dt - tau * om);
     const TE gm1 = -om * igm0;
     const TE gm2 = (-dt + TE(2) * tau * om - dt * ex) * igm0;
     const TE gm3 = -gm1;
@@ -170,11 +170,11 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
     TE hl[4], hr[4];
     {
         const TE sl = g_sqrt(L.lam), zl = sl * L.u, el = g_exp(-zl * zl);
-        const TE gl = TE(0.5) * isqpi * el / sl;
+        const TE gl = g_div(TE(0.5) * isqpi * el, sl);
         const TE h0 = half_erfc(-zl, el);
         half_tab<4>(L.u, L.i2l, h0, L.u * h0 + gl, hl);
         const TE sr = g_sqrt(R.lam), zr = sr * R.u, er = g_exp(-zr * zr);
-        const TE gr = TE(0.5) * isqpi * er / sr;
+        const TE gr = g_div(TE(0.5) * isqpi * er, sr);
         const TE r0 = half_erfc(zr, er);
         half_tab<4>(R.u, R.i2l, r0, R.u * r0 - gr, hr);
     }
@@ -240,11 +240,11 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         const TState<TS> st{(TS)E.u, (TS)E.i2l, (TS)(TE(2) * E.lam)};
         const TS cB = (TS)(gm2 * E.den), cF = (TS)(p2 * E.den);
         const TE se = g_sqrt(E.lam), ze = se * E.u, ee = g_exp(-ze * ze);
-        const TE ge = TE(0.5) * isqpi * ee / se;
+        const TE ge = g_div(TE(0.5) * isqpi * ee, se);
         // erfc(x) + erfc(-x) = 2: the u<0 half's zeroth moment is 1 - that of the u>0 half
         const TE hp0 = half_erfc(-ze, ee);
         const TE hm0 = TE(1) - hp0;
-        const TE c = TE(2) / (E.den * dx);
+        const TE c = TE(2) * g_rcp(E.den * dx);
         TS a[5], t[5];
         {   // u > 0 half with the left normal slope (GKMflux.cpp:115, 124, 170, 210)
             TS hp[7];
@@ -280,7 +280,7 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         const TE* h = side == 0 ? hl : hr;
         const TS V = (TS)S.v, W = (TS)S.w, den = (TS)S.den;
         const TState<TS> st{(TS)S.u, (TS)S.i2l, (TS)(TE(2) * S.lam)};
-        const TE iden = TE(1) / S.den;
+        const TE iden = g_rcp(S.den);
         TS ax[5], ay[5], az[5];
         slope_from_lab(in.D(side, 0) * iden, in.D(side, 1) * iden, in.D(side, 2) * iden, in.D(side, 3) * iden,
                        in.D(side, 4) * iden, S.v, S.w, st, inv_prn, kp3, ikp3, ax);
@@ -451,7 +451,7 @@ __host__ __device__ __forceinline__ void gks_flux_tf(const IN& in, TE tau, TE dt
         mom_uvw(fu + 1, st.th, kp4, t, Fs, 3);
         // time slope of E from den_e B_ (GKMflux.cpp:183-191), then the p1 term
         // (GKMflux.cpp:225) merged with the (V, W) part of the tangential terms
-        const TS iden = (TS)(TE(1) / E.den);
+        const TS iden = (TS)g_rcp(E.den);
         TS b[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) b[q] = Be[q] * iden;

@@ -39,6 +39,40 @@ template <> __host__ __device__ __forceinline__ float g_sqrt<float>(float x) { r
 template <> __host__ __device__ __forceinline__ float g_exp<float>(float x) { return expf(x); }
 template <> __host__ __device__ __forceinline__ float g_erfc<float>(float x) { return erfcf(x); }
 
+// 1/b and a/b.  FP64 device code skips the correctly rounded division's
+// operand scaling and fix-up (v_div_scale, v_div_fmas, v_div_fixup; 11 VALU
+// instructions on gfx950): the hardware reciprocal is refined by two Newton
+// steps r <- r + r (1 - b r), each squaring the relative error, and a/b takes
+// the residual correction q <- q + r (a - b q) of q = a r (Markstein): 5 and 8
+// instructions.  For normal operands with a normal quotient the results lie
+// within 1 ulp of the correctly rounded ones; a zero, infinite or NaN divisor
+// gives NaN (the flux's states are positive and finite, and a non-physical
+// state is caught by the CFL reduction).  Host code and other types divide.
+template <typename T> __host__ __device__ __forceinline__ T g_rcp(T b) { return T(1) / b; }
+template <typename T> __host__ __device__ __forceinline__ T g_div(T a, T b) { return a / b; }
+template <>
+__host__ __device__ __forceinline__ double g_rcp<double>(double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return fma(fma(-b, r, 1.0), r, r);
+#else
+    return 1.0 / b;
+#endif
+}
+template <>
+__host__ __device__ __forceinline__ double g_div<double>(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = g_rcp(b);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
+
 // 0.5 erfc(x) given E = exp(-x^2), which the half-space moments of a
 // Maxwellian form anyway (gkm_flux.hpp).  erfc(t) = E erfcx(t) for t = |x|,
 // erfc(-t) = 2 - erfc(t), and erfcx(t) = 2/(1 + 2t) P((t - 3)/(t + 3)) with P
@@ -90,7 +124,7 @@ __host__ __device__ __forceinline__ double half_erfc<double>(double x, double E)
     const double t = fabs(x);
     // one division for both y = (t - 3)/(t + 3) and 1/(1 + 2t)
     const double a = t + 3.0, b = 1.0 + 2.0 * t;
-    const double r = 1.0 / (a * b);
+    const double r = g_rcp(a * b);
     const double y = (t - 3.0) * b * r;
     double acc = kErfcxP[24];
 #pragma unroll
@@ -130,14 +164,14 @@ __host__ __device__ __forceinline__ T weno_side(T a, T b, T c, T d, T e, T f, T 
 
 // Pieces of weno_pair.  wmax: the larger of two indicators (fmax for float
 // and double: one instruction, equal to a > b ? a : b for non-NaN values).
-// wratio: the weights' normalising division; FP32 (mixed precision, device)
-// multiplies by the hardware reciprocal (1 ulp) instead of dividing.
+// wratio: the weights' normalising division (g_div); FP32 (mixed precision,
+// device) multiplies by the hardware reciprocal (1 ulp) instead of dividing.
 template <typename T>
 __host__ __device__ __forceinline__ T wmax(T a, T b) { return a > b ? a : b; }
 template <> __host__ __device__ __forceinline__ float wmax<float>(float a, float b) { return fmaxf(a, b); }
 template <> __host__ __device__ __forceinline__ double wmax<double>(double a, double b) { return fmax(a, b); }
 template <typename T>
-__host__ __device__ __forceinline__ T wratio(T n, T d) { return n / d; }
+__host__ __device__ __forceinline__ T wratio(T n, T d) { return g_div(n, d); }
 __device__ __forceinline__ float wratio(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
 
 // nonlinear combination of the four candidates (weno_side's second half)
@@ -193,7 +227,7 @@ __host__ __device__ __forceinline__ void weno_pair(T a, T b, T c, T d, T e, T f,
 template <typename T>
 __host__ __device__ __forceinline__ T temp_gr(const T W[5], T gR)
 {
-    const T ir = T(1) / W[0];
+    const T ir = g_rcp(W[0]);
     return gR * (W[4] - T(0.5) * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) * ir) * ir;
 }
 
@@ -220,21 +254,21 @@ __host__ __device__ __forceinline__ void weno5(const T S[6][5], const Gas<T>& g,
     T Tl, Tr;
     weno_pair(Tc[0], Tc[1], Tc[2], Tc[3], Tc[4], Tc[5], g.eps, Tl, Tr);
     const T Rg = g.R / (g.gam - T(1));
-    Wl[4] = Wl[0] * Rg * Tl + T(0.5) * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
-    Wr[4] = Wr[0] * Rg * Tr + T(0.5) * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+    Wl[4] = Wl[0] * Rg * Tl + g_div(T(0.5) * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]), Wl[0]);
+    Wr[4] = Wr[0] * Rg * Tr + g_div(T(0.5) * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]), Wr[0]);
 }
 
 // collision time mu(T)/p of the face's left cell, Sutherland law (update.cpp:276-278)
 template <typename T>
 __host__ __device__ __forceinline__ T collision_time(const T W[5], const Gas<T>& g)
 {
-    T ir = T(1) / W[0];
+    T ir = g_rcp(W[0]);
     T u = W[1] * ir, v = W[2] * ir, w = W[3] * ir;
     T P = (W[0] * (g.gam - T(1))) * (W[4] * ir - T(0.5) * (u * u + v * v + w * w));
-    T Tt = P / (g.R * W[0]);
-    T x = Tt / g.T0;
-    T mu = g.mu0 * (x * g_sqrt(x)) * (g.T0 + T(110.4)) / (Tt + T(110.4));
-    return mu / P;
+    T Tt = g_div(P, g.R * W[0]);
+    T x = g_div(Tt, g.T0);
+    T mu = g_div(g.mu0 * (x * g_sqrt(x)) * (g.T0 + T(110.4)), Tt + T(110.4));
+    return g_div(mu, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -254,13 +288,13 @@ template <typename T>
 __host__ __device__ __forceinline__ void maxw_init(Maxw<T>& s, const T W[5], const Gas<T>& g)
 {
     s.den = W[0];
-    T ir = T(1) / W[0];
+    T ir = g_rcp(W[0]);
     s.u = W[1] * ir;
     s.v = W[2] * ir;
     s.w = W[3] * ir;
     T P = (s.den * (g.gam - T(1))) * (W[4] * ir - T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w));
     s.i2l = P * ir;  // 1/(2 lambda) = p/rho
-    s.lam = T(0.5) / s.i2l;
+    s.lam = T(0.5) * g_rcp(s.i2l);
     s.e2 = T(g.K) * s.i2l;
     s.e4 = T(g.K * (g.K + 2)) * s.i2l * s.i2l;
 }
@@ -319,8 +353,8 @@ __host__ __device__ __forceinline__ void weno5_mixed(const double S[6][5], const
                      (float)(Tc[4] - Tc[2]), (float)(Tc[5] - Tc[2]), eps, l, r);
     const double Tl = Tc[2] + (double)l, Tr = Tc[2] + (double)r;
     const double Rg = g.R / (g.gam - 1.0);
-    Wl[4] = Wl[0] * Rg * Tl + 0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]) / Wl[0];
-    Wr[4] = Wr[0] * Rg * Tr + 0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]) / Wr[0];
+    Wl[4] = Wl[0] * Rg * Tl + g_div(0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]), Wl[0]);
+    Wr[4] = Wr[0] * Rg * Tr + g_div(0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]), Wr[0]);
 }
 
 __host__ __device__ __forceinline__ Gas<float> to_f32(const Gas<double>& g)

@@ -10,35 +10,110 @@ namespace lpt {
 
 __device__ __forceinline__ int wrapi(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
 
-// In-place natural-spline line solve.  The line holds f_0..f_{M-1} at
-// positions 0..M-1 (stride st) and receives c_0..c_{M+1}; Src reads f_p.
-template <typename Src>
-__device__ __forceinline__ void line_solve(int M, const Src& src, double* out, long st, const double* __restrict__ piv)
+// Natural-spline line solve in place.  The line holds f_0..f_{M-1} at
+// positions 0..M-1 (stride st) and receives c_0..c_{M+1}.  With n = M - 2
+// unknowns (row r reads f_{r+1} and writes position r + 2) the Thomas sweeps
+// are
+//   forward   dp_r = (6 f_{r+1} - [r = 0] c_1 - [r = n-1] c_M - dp_{r-1}) piv_r
+//   backward  x_{n-1} = dp_{n-1},  x_r = dp_r - piv_r x_{r+1},
+// then c_1 = f_0, c_M = f_{M-1}, c_0 = 2 c_1 - x_0, c_{M+1} = 2 c_M - x_{n-1}
+// (einspline natural boundary; piv_r = 1/(4 - piv_{r-1}), piv_0 = 1/4, formed on
+// the host).
+//
+// The rows go in blocks of B kept in registers, so every position is read once
+// and written once (the two-sweep form writes dp and reads it back).  Block j's
+// backward sweep enters with x_{r0+B}, obtained by running the backward
+// recurrence over block j+1 from x = dp at its last row: exact when that row
+// is n - 1, otherwise off by piv x_{r+1}, an error each later step multiplies
+// by piv_r <= 2 - sqrt(3) = 0.268, so by 0.268^32 = 5e-19 of the line's values
+// after the 32 rows of block j+1 (below rounding: results equal the sequential
+// sweeps' to a few ulp).  The forward sweep is the sequential one.
+template <int B, bool FULL>
+__host__ __device__ __forceinline__ void fwd_block(double (&D)[B], int r0, int n, const double* line, long st,
+                                          const double* __restrict__ piv, double c1, double cM, double& dp)
+{
+#pragma unroll
+    for (int k = 0; k < B; k++) D[k] = (FULL || r0 + k < n) ? line[(long)(r0 + k + 1) * st] : 0.0;
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+        const int r = r0 + k;
+        if (FULL || r < n) {
+            double rhs = 6.0 * D[k];
+            if (r == 0) rhs -= c1;
+            if (r == n - 1) rhs -= cM;
+            dp = (rhs - dp) * piv[r];
+            D[k] = dp;
+        }
+    }
+}
+
+// backward sweep over the rows of D (rows r0.. below n) entering with
+// x = x_{r0+B} (0 for the line's last block); WRITE stores x_r at position r + 2
+template <int B, bool FULL, bool WRITE>
+__host__ __device__ __forceinline__ double bwd_block(const double (&D)[B], int r0, int n, double x, double* line, long st,
+                                            const double* __restrict__ piv)
+{
+#pragma unroll
+    for (int k = B - 1; k >= 0; k--) {
+        const int r = r0 + k;
+        if (FULL || r < n) {
+            x = D[k] - piv[r] * x;
+            if (WRITE) line[(long)(r + 2) * st] = x;
+        }
+    }
+    return x;
+}
+
+// block j of D (rows r0..) with the next block in E (rows r0 + B..): E is read
+// forward first, then swept backward as the warm-up of D's backward sweep
+template <int B>
+__host__ __device__ __forceinline__ double pair_step(double (&D)[B], double (&E)[B], int r0, int n, double* line, long st,
+                                            const double* __restrict__ piv, double c1, double cM, double& dp)
+{
+    double x;
+    if (r0 + 2 * B <= n) {
+        fwd_block<B, true>(E, r0 + B, n, line, st, piv, c1, cM, dp);
+        x = bwd_block<B, true, false>(E, r0 + B, n, 0.0, line, st, piv);
+    } else {
+        fwd_block<B, false>(E, r0 + B, n, line, st, piv, c1, cM, dp);
+        x = bwd_block<B, false, false>(E, r0 + B, n, 0.0, line, st, piv);
+    }
+    return bwd_block<B, true, true>(D, r0, n, x, line, st, piv);
+}
+
+template <int B>
+__host__ __device__ __forceinline__ void line_solve_blocked(int M, double* line, long st, const double* __restrict__ piv)
 {
     const int n = M - 2;
-    const double c1 = src(0), cM = src(M - 1);
-    double next = src(1);
-    double dp = 0.0;
-    for (int r = 0; r < n; r++) {
-        const double f = next;
-        if (r + 1 < n) next = src(r + 2);  // read before position r+2 is overwritten
-        double rhs = 6.0 * f;
-        if (r == 0) rhs -= c1;
-        if (r == n - 1) rhs -= cM;
-        dp = (rhs - dp) * piv[r];
-        out[(long)(r + 2) * st] = dp;
+    const double c1 = line[0], cM = line[(long)(M - 1) * st];
+    double P[B], Q[B];
+    double dp = 0.0, x0 = 0.0;
+    if (B <= n) fwd_block<B, true>(P, 0, n, line, st, piv, c1, cM, dp);
+    else fwd_block<B, false>(P, 0, n, line, st, piv, c1, cM, dp);
+    for (int r0 = 0;; r0 += 2 * B) {
+        // P holds rows r0..
+        if (r0 + B >= n) {
+            const double x = bwd_block<B, false, true>(P, r0, n, 0.0, line, st, piv);
+            if (r0 == 0) x0 = x;
+            break;
+        }
+        const double x = pair_step<B>(P, Q, r0, n, line, st, piv, c1, cM, dp);
+        if (r0 == 0) x0 = x;
+        // Q holds rows r0 + B..
+        if (r0 + 2 * B >= n) {
+            bwd_block<B, false, true>(Q, r0 + B, n, 0.0, line, st, piv);
+            break;
+        }
+        pair_step<B>(Q, P, r0 + B, n, line, st, piv, c1, cM, dp);
     }
-    double x = dp;  // x_{n-1}
-    const double xlast = x;
-    for (int r = n - 2; r >= 0; r--) {
-        x = out[(long)(r + 2) * st] - piv[r] * x;
-        out[(long)(r + 2) * st] = x;
-    }
-    out[st] = c1;
-    out[(long)M * st] = cM;
-    out[0] = 2.0 * c1 - x;                        // x = c_2
-    out[(long)(M + 1) * st] = 2.0 * cM - xlast;   // xlast = c_{M-1}
+    line[st] = c1;
+    line[(long)M * st] = cM;
+    line[0] = 2.0 * c1 - x0;                       // x0 = c_2
+    line[(long)(M + 1) * st] = 2.0 * cM - dp;      // dp = dp_{n-1} = x_{n-1} = c_{M-1}
 }
+
+// rows per register block of line_solve_blocked
+constexpr int kLineBlock = 32;
 
 // Pass 1: z lines straight from the state (periodic wrap).  Source field q as
 // generateSplines forms it (update.cpp:1202-1216): 0 = rho, 1..3 = velocity
@@ -59,13 +134,13 @@ __device__ __forceinline__ double field_at(const double* __restrict__ W, long pl
 // ---------------------------------------------------------------------------
 // z pass: one wave per z line.  Lane l of chunk j holds knot position
 // 64 j + l, so loads and stores are contiguous along z.  Both sweeps of
-// line_solve are first-order linear recurrences,
+// the natural-spline solve (line_solve_blocked) are first-order linear recurrences,
 //   forward   dp_r = -piv_r dp_{r-1} + piv_r rhs_r      (dp_{-1} = 0)
 //   backward  x_r  = -piv_r x_{r+1}  + dp_r             (x_{n-1} = dp_{n-1}),
 // so each 64-position chunk is an inclusive wave scan of the affine maps
 // x -> A x + B (DPP row shifts and row broadcasts, no LDS) applied to the
 // carry from the previous chunk.  The backward sweep scans lane-reversed
-// chunks.  Results equal line_solve's to rounding (the scan associates the
+// chunks.  Results equal the sequential sweeps' to rounding (the scan associates the
 // same products differently: measured <= 4 ulp of the line's largest value).
 // ---------------------------------------------------------------------------
 template <int CTRL, int ROW>
@@ -104,7 +179,7 @@ __device__ __forceinline__ double lane_of(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// line_solve of one z line: f holds the M knot values (zero beyond M), out
+// natural-spline solve of one z line: f holds the M knot values (zero beyond M), out
 // receives the M + 2 coefficients.  Called by every lane of a full wave.
 template <int K>
 __device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double* __restrict__ piv,
@@ -221,9 +296,7 @@ __device__ __forceinline__ void prefilter_inplace_one(long t, double* __restrict
         base = t;
         st = (long)g.N * g.N;
     }
-    double* line = coef + base;
-    auto src = [&](int p) { return line[(long)p * st]; };
-    line_solve(g.nt, src, line, st, piv);
+    line_solve_blocked<kLineBlock>(g.nt, coef + base, st, piv);
 }
 
 // ---------------------------------------------------------------------------

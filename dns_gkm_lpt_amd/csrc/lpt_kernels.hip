@@ -90,7 +90,8 @@ static hipError_t with_chunks(int k, F&& f)
     }
 }
 
-__global__ void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis, const double* __restrict__ piv)
+__global__ __launch_bounds__(64) void prefilter_inplace_kernel(double* __restrict__ coef, Geom g, int axis,
+                                                                const double* __restrict__ piv)
 {
     prefilter_inplace_one((long)blockIdx.x * blockDim.x + threadIdx.x, coef, g, axis, piv);
 }

@@ -172,6 +172,54 @@ __device__ __forceinline__ void wave_scan(double& A, double& B)
     scan_step<0x143, 0xc>(A, B);  // row_bcast:31 into rows 2 and 3
 }
 
+// largest knot count per line a wave holds (17 chunks of 64: nc <= 1082)
+constexpr int kMaxZChunks = 17;
+
+// Chunks away from both ends of a line see one pivot: piv_r = 1/(4 - piv_{r-1})
+// reaches its FP64 fixed point pc (2 - sqrt(3) rounded) at r = 13 and stays
+// there, so the maps of such a chunk are x -> a x + B_l with one a = -pc.  The
+// scan then carries B alone, B_l += a^k B_{l-k}, with a^k a constant per step
+// (a^1, a^2, a^4, a^8 for the row shifts; a^(l mod 16 + 1) and a^(l - 31) per
+// lane for the row broadcasts), and the carry enters with a^(l + 1).
+struct ScanW {
+    double a1, a2, a4, a8, w15, w31, wc;
+};
+
+__device__ __forceinline__ double pow_small(double a, int e)  // a^e, 0 <= e <= 64, by squaring
+{
+    double r = 1.0;
+    for (; e; e >>= 1, a *= a)
+        if (e & 1) r *= a;
+    return r;
+}
+
+__device__ __forceinline__ ScanW scan_weights(double a, int lane)
+{
+    ScanW w;
+    w.a1 = a;
+    w.a2 = a * a;
+    w.a4 = w.a2 * w.a2;
+    w.a8 = w.a4 * w.a4;
+    const int row = lane >> 4;
+    w.w15 = (row & 1) ? pow_small(a, (lane & 15) + 1) : 0.0;
+    w.w31 = row >= 2 ? pow_small(a, lane - 31) : 0.0;
+    w.wc = pow_small(a, lane + 1);
+    return w;
+}
+
+// inclusive scan over the 64 lanes of maps x -> a x + B with a common a (see
+// above): returns sum_{m <= l} a^(l-m) B_m at lane l; all lanes active
+__device__ __forceinline__ double wave_scan_const(double B, const ScanW& w)
+{
+    B += w.a1 * dpp64<0x111, 0xf>(0.0, B);   // row_shr:1
+    B += w.a2 * dpp64<0x112, 0xf>(0.0, B);   // row_shr:2
+    B += w.a4 * dpp64<0x114, 0xf>(0.0, B);   // row_shr:4
+    B += w.a8 * dpp64<0x118, 0xf>(0.0, B);   // row_shr:8
+    B += w.w15 * dpp64<0x142, 0xa>(0.0, B);  // row_bcast:15 into rows 1 and 3
+    B += w.w31 * dpp64<0x143, 0xc>(0.0, B);  // row_bcast:31 into rows 2 and 3
+    return B;
+}
+
 __device__ __forceinline__ double lane_of(double v, int l)
 {
     const long long x = __double_as_longlong(v);
@@ -186,6 +234,14 @@ __device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double*
                                             double* __restrict__ out)
 {
     const int lane = threadIdx.x & 63, n = M - 2;
+    // chunk j (rows 64 j - 1 .. 64 j + 62) is "inner" when j >= 1 (rows >= 63,
+    // past the pivots' convergence) and its last row is below n - 1 (no end
+    // condition, no row past the line); lines of at most two chunks have none,
+    // and lines of 17 chunks (over 576 knots) keep the general scan, with which
+    // their kernel needs no scratch
+    constexpr bool kInner = K >= 3 && K < kMaxZChunks;
+    const double pc = kInner ? piv[n - 1 < 63 ? n - 1 : 63] : 0.0;
+    const ScanW sw = scan_weights(-pc, lane);
     const double c1 = lane_of(f[0], 0);
     double cM = 0.0;
 #pragma unroll
@@ -194,6 +250,11 @@ __device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double*
     double carry = 0.0;
 #pragma unroll
     for (int j = 0; j < K; j++) {  // forward: r = p - 1 for position p = 64 j + lane
+        if (kInner && j >= 1 && 64 * j + 62 <= n - 2) {
+            f[j] = sw.wc * carry + wave_scan_const(pc * (6.0 * f[j]), sw);
+            carry = lane_of(f[j], 63);
+            continue;
+        }
         const int r = 64 * j + lane - 1;
         double A = 1.0, B = 0.0;
         if (r >= 0 && r < n) {
@@ -218,6 +279,12 @@ __device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double*
     for (int j = K - 1; j >= 0; j--) {  // backward on reversed chunks: lane l holds r = 64 j + 62 - l
         const double d = __shfl(f[j], 63 - lane);
         const int r = 64 * j + 62 - lane;
+        if (kInner && j >= 1 && 64 * j + 62 <= n - 2) {
+            const double x = sw.wc * carry + wave_scan_const(d, sw);
+            carry = lane_of(x, 63);
+            out[r + 2] = x;
+            continue;
+        }
         double A = 1.0, B = 0.0;
         if (r >= 0 && r < n) {
             A = r == n - 1 ? 0.0 : -piv[r];
@@ -237,8 +304,6 @@ __device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double*
     }
 }
 
-// largest knot count per line a wave holds (17 chunks of 64: nc <= 1082)
-constexpr int kMaxZChunks = 17;
 
 template <int K>
 __device__ __forceinline__ void prefilter_z_line(long line, const ZArgs& z)

@@ -20,14 +20,15 @@ __device__ __forceinline__ int wrapi(int x, int n) { return x < 0 ? x + n : (x >
 // (einspline natural boundary; piv_r = 1/(4 - piv_{r-1}), piv_0 = 1/4, formed on
 // the host).
 //
-// The rows go in blocks of B kept in registers, so every position is read once
-// and written once (the two-sweep form writes dp and reads it back).  Block j's
-// backward sweep enters with x_{r0+B}, obtained by running the backward
-// recurrence over block j+1 from x = dp at its last row: exact when that row
-// is n - 1, otherwise off by piv x_{r+1}, an error each later step multiplies
-// by piv_r <= 2 - sqrt(3) = 0.268, so by 0.268^32 = 5e-19 of the line's values
-// after the 32 rows of block j+1 (below rounding: results equal the sequential
-// sweeps' to a few ulp).  The forward sweep is the sequential one.
+// The rows go in blocks of B, three kept in registers, so every position is
+// read once and written once (the two-sweep form writes dp and reads it back).
+// Block j's backward sweep enters with x_{r0+B}, obtained by running the
+// backward recurrence over blocks j+2 and j+1 from x = dp at the last row of
+// block j+2: exact when that row is n - 1, otherwise off by piv x_{r+1}, an
+// error each later step multiplies by piv_r <= 2 - sqrt(3) = 0.268, so by
+// 0.268^32 = 5e-19 of the line's values after the 2B = 32 rows of the two
+// blocks (below rounding: results equal the sequential sweeps' to a few ulp).
+// The forward sweep is the sequential one.
 template <int B, bool FULL>
 __host__ __device__ __forceinline__ void fwd_block(double (&D)[B], int r0, int n, const double* line, long st,
                                           const double* __restrict__ piv, double c1, double cM, double& dp)
@@ -64,21 +65,36 @@ __host__ __device__ __forceinline__ double bwd_block(const double (&D)[B], int r
     return x;
 }
 
-// block j of D (rows r0..) with the next block in E (rows r0 + B..): E is read
-// forward first, then swept backward as the warm-up of D's backward sweep
+// One step of the ring: D holds rows r0.., E rows r0 + B.. (when r0 + B < n);
+// F receives rows r0 + 2B.. (read forward here), then F and E are swept
+// backward as the warm-up of D's backward sweep, which writes D's rows.  Returns
+// true when D was the line's last block.
 template <int B>
-__host__ __device__ __forceinline__ double pair_step(double (&D)[B], double (&E)[B], int r0, int n, double* line, long st,
-                                            const double* __restrict__ piv, double c1, double cM, double& dp)
+__host__ __device__ __forceinline__ bool ring_step(double (&D)[B], double (&E)[B], double (&F)[B], int r0, int n,
+                                                   double* line, long st, const double* __restrict__ piv, double c1,
+                                                   double cM, double& dp, double& x0)
 {
     double x;
-    if (r0 + 2 * B <= n) {
-        fwd_block<B, true>(E, r0 + B, n, line, st, piv, c1, cM, dp);
-        x = bwd_block<B, true, false>(E, r0 + B, n, 0.0, line, st, piv);
-    } else {
-        fwd_block<B, false>(E, r0 + B, n, line, st, piv, c1, cM, dp);
-        x = bwd_block<B, false, false>(E, r0 + B, n, 0.0, line, st, piv);
+    if (r0 + B >= n) {  // D holds the end of the line
+        x = bwd_block<B, false, true>(D, r0, n, 0.0, line, st, piv);
+        if (r0 == 0) x0 = x;
+        return true;
     }
-    return bwd_block<B, true, true>(D, r0, n, x, line, st, piv);
+    if (r0 + 2 * B >= n) {  // E holds the end of the line: its backward sweep starts exactly at n - 1
+        x = bwd_block<B, false, false>(E, r0 + B, n, 0.0, line, st, piv);
+    } else {
+        if (r0 + 3 * B <= n) {
+            fwd_block<B, true>(F, r0 + 2 * B, n, line, st, piv, c1, cM, dp);
+            x = bwd_block<B, true, false>(F, r0 + 2 * B, n, 0.0, line, st, piv);
+        } else {
+            fwd_block<B, false>(F, r0 + 2 * B, n, line, st, piv, c1, cM, dp);
+            x = bwd_block<B, false, false>(F, r0 + 2 * B, n, 0.0, line, st, piv);
+        }
+        x = bwd_block<B, true, false>(E, r0 + B, n, x, line, st, piv);
+    }
+    x = bwd_block<B, true, true>(D, r0, n, x, line, st, piv);
+    if (r0 == 0) x0 = x;
+    return false;
 }
 
 template <int B>
@@ -86,25 +102,18 @@ __host__ __device__ __forceinline__ void line_solve_blocked(int M, double* line,
 {
     const int n = M - 2;
     const double c1 = line[0], cM = line[(long)(M - 1) * st];
-    double P[B], Q[B];
+    double P[B], Q[B], R[B];
     double dp = 0.0, x0 = 0.0;
     if (B <= n) fwd_block<B, true>(P, 0, n, line, st, piv, c1, cM, dp);
     else fwd_block<B, false>(P, 0, n, line, st, piv, c1, cM, dp);
-    for (int r0 = 0;; r0 += 2 * B) {
-        // P holds rows r0..
-        if (r0 + B >= n) {
-            const double x = bwd_block<B, false, true>(P, r0, n, 0.0, line, st, piv);
-            if (r0 == 0) x0 = x;
-            break;
-        }
-        const double x = pair_step<B>(P, Q, r0, n, line, st, piv, c1, cM, dp);
-        if (r0 == 0) x0 = x;
-        // Q holds rows r0 + B..
-        if (r0 + 2 * B >= n) {
-            bwd_block<B, false, true>(Q, r0 + B, n, 0.0, line, st, piv);
-            break;
-        }
-        pair_step<B>(Q, P, r0 + B, n, line, st, piv, c1, cM, dp);
+    if (B < n) {
+        if (2 * B <= n) fwd_block<B, true>(Q, B, n, line, st, piv, c1, cM, dp);
+        else fwd_block<B, false>(Q, B, n, line, st, piv, c1, cM, dp);
+    }
+    for (int r0 = 0;; r0 += 3 * B) {
+        if (ring_step<B>(P, Q, R, r0, n, line, st, piv, c1, cM, dp, x0)) break;
+        if (ring_step<B>(Q, R, P, r0 + B, n, line, st, piv, c1, cM, dp, x0)) break;
+        if (ring_step<B>(R, P, Q, r0 + 2 * B, n, line, st, piv, c1, cM, dp, x0)) break;
     }
     line[st] = c1;
     line[(long)M * st] = cM;
@@ -112,8 +121,9 @@ __host__ __device__ __forceinline__ void line_solve_blocked(int M, double* line,
     line[(long)(M + 1) * st] = 2.0 * cM - dp;      // dp = dp_{n-1} = x_{n-1} = c_{M-1}
 }
 
-// rows per register block of line_solve_blocked
-constexpr int kLineBlock = 32;
+// rows per register block of line_solve_blocked (three blocks in registers:
+// the warm-up spans two)
+constexpr int kLineBlock = 16;
 
 // Pass 1: z lines straight from the state (periodic wrap).  Source field q as
 // generateSplines forms it (update.cpp:1202-1216): 0 = rho, 1..3 = velocity

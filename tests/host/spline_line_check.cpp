@@ -7,8 +7,8 @@
 // Lines of 4 .. 1030 knots (fewer than one register block, exact multiples of
 // it and ragged ends), strides 1 and 7, random and smooth data.
 //   spline_line_check <residual tol> <sweep tol>    (relative to max |f|)
-// Built with -DLINE_BLOCK=8 the warm-up spans 8 rows instead of 32 (its error
-// 0.268^8 = 2.7e-5 of the line's values), which the check must reject.
+// Built with -DLINE_BLOCK=8 the warm-up spans 16 rows instead of 32 (its error
+// 0.268^16 = 7e-10 of the line's values), which the check must reject.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>

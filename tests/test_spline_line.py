@@ -1,15 +1,15 @@
 """The blocked natural-spline line solve of the x/y prefilter passes
 (csrc/lpt_device.hpp line_solve_blocked) on the host: every position read and
-written once, the backward sweep of each 32-row register block entering from
-a warm-up over the next block.
+written once, three 16-row blocks in registers, the backward sweep of each
+block entering from a warm-up over the next two (32 rows).
 
 Checked against the definition of the natural cubic spline (knot values
 (c_i + 4 c_{i+1} + c_{i+2})/6 = f_i and zero second differences at both ends;
 residual <= 1e-14 of max |f|, measured 3.5e-16) and against the plain
 two-sweep Thomas solve (<= 1e-15, measured 1.1e-16) on 60 lines: 4 .. 1030
 knots (shorter than one block, exact multiples of it, ragged ends), strides 1
-and 7, random and smooth data.  The same check rejects a warm-up of 8 rows
-(error 0.268^8 of the line's values)."""
+and 7, random and smooth data.  The same check rejects blocks of 8 rows, a
+warm-up of 16 (error 0.268^16 of the line's values; measured 4.6e-10)."""
 import os
 import shutil
 import subprocess

@@ -44,7 +44,7 @@ __host__ __device__ __forceinline__ void maxw_vjp(const double W[5], const Maxw<
                                                   const MaxwBar& b, double Wb[5])
 {
     // theta = (gam - 1) (W4 / W0 - |W1..3|^2 / (2 W0^2)), u = W1 / W0, ...
-    const double ir = 1.0 / W[0];
+    const double ir = g_rcp(W[0]);
     const double t = (gam - 1.0) * theta_bar(b, s);
     Wb[0] += b.den + (t * (s.u * s.u + s.v * s.v + s.w * s.w - W[4] * ir) - (s.u * b.u + s.v * b.v + s.w * b.w)) * ir;
     Wb[1] += (b.u - t * s.u) * ir;
@@ -177,12 +177,12 @@ __host__ __device__ __forceinline__ void half_start_vjp(double u, double lam, do
     const double pi = 3.141592653589793238462643383279502884;
     const double isqpi = 0.5641895835477562869480794515607726;  // 1/sqrt(pi)
     const double sl = g_sqrt(lam), e = g_exp(-lam * u * u);
-    const double gsg = sgn * 0.5 * e / g_sqrt(lam * pi);  // the sgn-signed exponential term of h1
+    const double gsg = g_div(sgn * 0.5 * e, g_sqrt(lam * pi));  // the sgn-signed exponential term of h1
     const double h0t = h0b + u * h1b;                     // h0's total adjoint
     const double zb = h0t * sgn * e * isqpi;              // of sqrt(lam) u
     const double gb = gsg * h1b;
     ub += h0 * h1b + sl * zb - 2.0 * lam * u * gb;
-    lamb += 0.5 * u * zb / sl - (u * u + 0.5 / lam) * gb;
+    lamb += g_div(0.5 * u * zb, sl) - (u * u + 0.5 * g_rcp(lam)) * gb;
 }
 
 __host__ __device__ __forceinline__ double dot5(const double a[5], const double b[5])
@@ -200,9 +200,9 @@ template <typename T>
 __host__ __device__ __forceinline__ TimeCoefs<T> time_coefs(T tau, T dt)
 {
     TimeCoefs<T> c;
-    const T ex = g_exp(-dt / tau);
+    const T ex = g_exp(-g_div(dt, tau));
     const T om = T(1) - ex;
-    const T igm0 = T(1) / (dt - tau * om);
+    const T igm0 = g_rcp(dt - tau * om);
     c.gm1 = -om * igm0;
     c.gm2 = (-dt + T(2) * tau * om - dt * ex) * igm0;
     c.gm3 = -c.gm1;
@@ -243,7 +243,7 @@ __host__ __device__ __forceinline__ void side_fwd(const Maxw<double>& S, double 
                                                   SideFwd& s)
 {
     s.st = TState<double>{S.u, S.i2l, 2.0 * S.lam};
-    s.iden = 1.0 / S.den;
+    s.iden = g_rcp(S.den);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
@@ -454,7 +454,7 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
     maxw_init(L, Wl, g);
     maxw_init(R, Wr, g);
     const double pl = 2.0 * L.den * L.i2l, pr = 2.0 * R.den * R.i2l;  // rho / lambda, as the flux forms it
-    const double taue = tau0 + dt * (::fabs(pl - pr) / ::fabs(pl + pr));
+    const double taue = tau0 + dt * g_div(::fabs(pl - pr), ::fabs(pl + pr));
     const TimeCoefs<Dual> tcd = time_coefs<Dual>(Dual(taue, 1.0), Dual(dt));
     const TimeCoefs<double> tc{tcd.gm1.v, tcd.gm2.v, tcd.gm3.v, tcd.gm4.v, tcd.gm5.v, tcd.p0.v,
                                tcd.p1.v,  tcd.p2.v,  tcd.p3.v,  tcd.p4.v,  tcd.p5.v};
@@ -463,11 +463,11 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
     double hl[4], hr[4];
     {
         const double sl = g_sqrt(L.lam), zl = sl * L.u, el = g_exp(-zl * zl);
-        const double gl = 0.5 * isqpi * el / sl;
+        const double gl = g_div(0.5 * isqpi * el, sl);
         const double h0 = half_erfc(-zl, el);
         half_tab<4>(L.u, L.i2l, h0, L.u * h0 + gl, hl);
         const double sr = g_sqrt(R.lam), zr = sr * R.u, er = g_exp(-zr * zr);
-        const double gr = 0.5 * isqpi * er / sr;
+        const double gr = g_div(0.5 * isqpi * er, sr);
         const double r0 = half_erfc(zr, er);
         half_tab<4>(R.u, R.i2l, r0, R.u * r0 - gr, hr);
     }
@@ -523,10 +523,10 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
     }
     // equilibrium halves with the cells' normal slopes
     const double se = g_sqrt(E.lam), ze = se * E.u, ee = g_exp(-ze * ze);
-    const double ge = 0.5 * isqpi * ee / se;
+    const double ge = g_div(0.5 * isqpi * ee, se);
     const double hp0 = half_erfc(-ze, ee);
     const double hm0 = 1.0 - hp0;
-    const double cn = 2.0 / (E.den * dx);
+    const double cn = 2.0 * g_rcp(E.den * dx);
     const double cBE = tc.gm2 * E.den, cFE = tc.p2 * E.den;
     auto normal_fwd = [&](int half, double hh[7], double dn[5], double m[5], double a[5]) {
         if (half == 0) half_tab<7>(sE.u, sE.th, hp0, E.u * hp0 + ge, hh);
@@ -566,7 +566,7 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
         for (int q = 0; q < 5; q++) t[q] = cFE * azE[q];
         mom_uvw(fuE + 1, sE.th, k.kp4, t, Fs, 3);
     }
-    const double idenE = 1.0 / E.den;
+    const double idenE = g_rcp(E.den);
     for (int q = 0; q < 5; q++) bE[q] = Be[q] * idenE;
     slope_t(bE, sE, k.inv_prn, k.kp3, k.ikp3, aE);
     const double c1 = tc.p1 * E.den;
@@ -777,12 +777,14 @@ __host__ __device__ inline void face_flux_vjp(const double x[50], double dt, dou
         const double sA = A > 0 ? 1.0 : (A < 0 ? -1.0 : 0.0), sB = B > 0 ? 1.0 : (B < 0 ? -1.0 : 0.0);
         const double aB = ::fabs(B);
         const double fb = dt * taueb;
-        const double Ab = fb * sA / aB, Bb = -fb * (::fabs(A) / aB) * sB / aB;
+        const double iaB = g_rcp(aB);
+        const double Ab = fb * sA * iaB, Bb = -fb * (::fabs(A) * iaB) * sB * iaB;
         const double plb = Ab + Bb, prb = Bb - Ab;
-        Lb.den += plb / L.lam;
-        Lb.lam -= plb * pl / L.lam;
-        Rb.den += prb / R.lam;
-        Rb.lam -= prb * pr / R.lam;
+        const double ill = g_rcp(L.lam), ilr = g_rcp(R.lam);
+        Lb.den += plb * ill;
+        Lb.lam -= plb * pl * ill;
+        Rb.den += prb * ilr;
+        Rb.lam -= prb * pr * ilr;
     }
     maxw_vjp(Wl, L, g.gam, Lb, xbar + 10);
     maxw_vjp(Wr, R, g.gam, Rb, xbar + 15);

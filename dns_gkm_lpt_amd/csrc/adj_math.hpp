@@ -42,9 +42,9 @@ __host__ __device__ __forceinline__ void weno_side_grad(double a, double b, doub
     const double p01 = D0 * D1, p23 = D2 * D3;
     const double n0 = 0.05 * D1 * p23, n1 = 0.45 * D0 * p23, n2 = 0.45 * p01 * D3, n3 = 0.05 * p01 * D2;
     const double Dn = n0 + n1 + n2 + n3;
-    const double out = (n0 * q0 + n1 * q1 + n2 * q2 + n3 * q3) / Dn;
+    const double out = g_div(n0 * q0 + n1 * q1 + n2 * q2 + n3 * q3, Dn);  // as wratio
     // out = N / Dn
-    const double iDn = wbar / Dn;
+    const double iDn = g_div(wbar, Dn);
     const double q0b = n0 * iDn, q1b = n1 * iDn, q2b = n2 * iDn, q3b = n3 * iDn;
     const double n0b = (q0 - out) * iDn, n1b = (q1 - out) * iDn, n2b = (q2 - out) * iDn, n3b = (q3 - out) * iDn;
     double D0b = 0.45 * (n1b * p23 + n2b * D1 * D3) + 0.05 * n3b * D1 * D2;

@@ -14,6 +14,8 @@
 
 #include <cmath>
 
+#include "gkm_math.hpp"
+
 namespace gkm {
 
 struct Dual {
@@ -31,8 +33,8 @@ __host__ __device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}
 __host__ __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 __host__ __device__ __forceinline__ Dual operator/(Dual a, Dual b)
 {
-    const double q = a.v / b.v;
-    return {q, (a.d - q * b.d) / b.v};
+    const double q = g_div(a.v, b.v);
+    return {q, (a.d - q * b.d) * g_rcp(b.v)};
 }
 __host__ __device__ __forceinline__ Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
 __host__ __device__ __forceinline__ Dual& operator-=(Dual& a, Dual b) { return a = a - b; }

@@ -174,8 +174,8 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
     for (int i = lo; i < hi; i++) {
         const int e = lpt::ens_entry(a.pfirst, a.npart, i);
         const lpt::EnsPart& P = a.sample[e];
-        const long p = (long)(i - a.pfirst[e]) * 256 + tid;
-        if (p < P.np) lpt::sample_one(p, P.pos, P.c[0], P.c[1], P.c[2], P.g, P.U, P.G);
+        const long t = (long)(i - a.pfirst[e]) * 256 + tid;  // four threads per tracer
+        if ((t >> 2) < P.np) lpt::sample_quad(t >> 2, tid & 3, P.pos, P.c[0], P.c[1], P.c[2], P.g, P.U, P.G);
     }
     if (sync()) return;
     // 2: dt of every member
@@ -223,8 +223,8 @@ __global__ __launch_bounds__(256, 2) void ens_step_kernel(PersistStep a)
         for (int i = lo; i < hi; i++) {
             const int e = lpt::ens_entry(a.pfirst, a.npart, i);
             const lpt::EnsPart& P = a.heun[e];
-            const long p = (long)(i - a.pfirst[e]) * 256 + tid;
-            if (p < P.np) lpt::heun_one(p, P.pos, P.U, P.c[0], P.c[1], P.c[2], P.g, *P.dt);
+            const long t = (long)(i - a.pfirst[e]) * 256 + tid;  // four threads per tracer
+            if ((t >> 2) < P.np) lpt::heun_quad(t >> 2, tid & 3, P.pos, P.U, P.c[0], P.c[1], P.c[2], P.g, *P.dt);
         }
     }
     if (a.trace) {  // the end of the last phase

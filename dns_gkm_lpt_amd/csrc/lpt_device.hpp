@@ -412,55 +412,77 @@ __device__ __forceinline__ Loc locate(double p, const Geom& g, bool deriv)
     return L;
 }
 
-__device__ __forceinline__ void sample_one(long p, const double* __restrict__ pos, const double* __restrict__ cu,
-                                           const double* __restrict__ cv, const double* __restrict__ cw, const Geom& g,
-                                           double* __restrict__ U, double* __restrict__ G)
+// The tricubic gathers of a tracer are done by the four lanes of a quad
+// (k = lane & 3; the whole quad calls together, with the same p): lane k reads
+// element k of each of the stencil's 4 x 4 rows of 4 coefficients along z, so
+// one load instruction of a wave touches the cache lines of 16 tracers rather
+// than 64 (one line per lane and load before); each lane sums its column over
+// the 16 rows and two quad exchanges add the four columns with their z
+// weights.  The sums are the same terms as eval_UBspline_3d_d_vg's, grouped
+// column first; every lane of the quad ends with the same values.
+__device__ __forceinline__ double quad_sum(double v)
+{
+    v += dpp64<0xB1, 0xf>(0.0, v);         // quad_perm [1, 0, 3, 2]
+    return v + dpp64<0x4E, 0xf>(0.0, v);  // quad_perm [2, 3, 0, 1]
+}
+
+__device__ __forceinline__ double pick4(const double w[4], int k)
+{
+    return k == 0 ? w[0] : (k == 1 ? w[1] : (k == 2 ? w[2] : w[3]));
+}
+
+__device__ __forceinline__ void sample_quad(long p, int k, const double* __restrict__ pos,
+                                            const double* __restrict__ cu, const double* __restrict__ cv,
+                                            const double* __restrict__ cw, const Geom& g, double* __restrict__ U,
+                                            double* __restrict__ G)
 {
     if (g.drop && isnan(pos[3 * p])) {  // a dropped tracer contributes nothing
+        if (k == 0) {
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
-            if (U) U[3 * p + f] = 0.0;
-            if (G) G[9 * p + 3 * f + 0] = G[9 * p + 3 * f + 1] = G[9 * p + 3 * f + 2] = 0.0;
+            for (int f = 0; f < 3; f++) {
+                if (U) U[3 * p + f] = 0.0;
+                if (G) G[9 * p + 3 * f + 0] = G[9 * p + 3 * f + 1] = G[9 * p + 3 * f + 2] = 0.0;
+            }
         }
         return;
     }
     const Loc Ly = locate(pos[3 * p + 0], g, true);
     const Loc Lx = locate(pos[3 * p + 1], g, true);
     const Loc Lz = locate(pos[3 * p + 2], g, true);
+    const double wz = pick4(Lz.w, k), dz = pick4(Lz.d, k);
     const long N = g.N;
+    const long c0 = ((long)Ly.i * N + Lx.i) * N + Lz.i + k;
     const double* cs[3] = {cu, cv, cw};
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        double v = 0, gy = 0, gx = 0, gz = 0;
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                const double* row = cs[f] + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
-                double rv = 0, rd = 0;
+        double P = 0, Py = 0, Px = 0;  // column k of the value, d/dy and d/dx sums
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    rv += Lz.w[c] * row[c];
-                    rd += Lz.d[c] * row[c];
-                }
-                const double wab = Ly.w[a] * Lx.w[b];
-                v += wab * rv;
-                gz += wab * rd;
-                gy += Ly.d[a] * Lx.w[b] * rv;
-                gx += Ly.w[a] * Lx.d[b] * rv;
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const double c = cs[f][c0 + ((long)a * N + b) * N];
+                P += (Ly.w[a] * Lx.w[b]) * c;
+                Py += (Ly.d[a] * Lx.w[b]) * c;
+                Px += (Ly.w[a] * Lx.d[b]) * c;
             }
-        if (U) U[3 * p + f] = v;
-        if (G) {
-            G[9 * p + 3 * f + 0] = gy;
-            G[9 * p + 3 * f + 1] = gx;
-            G[9 * p + 3 * f + 2] = gz;
+        const double v = quad_sum(wz * P), gy = quad_sum(wz * Py), gx = quad_sum(wz * Px), gz = quad_sum(dz * P);
+        if (k == 0) {
+            if (U) U[3 * p + f] = v;
+            if (G) {
+                G[9 * p + 3 * f + 0] = gy;
+                G[9 * p + 3 * f + 1] = gx;
+                G[9 * p + 3 * f + 2] = gz;
+            }
         }
     }
 }
 
 // Heun RK2 of Particle::updatePosition (ParticleTrack.cpp:48-73): predictor
 // x + U dt (not wrapped) sampled on the new field, corrector, wrap to [0, 2pi].
-__device__ __forceinline__ void heun_one(long p, double* __restrict__ pos, const double* __restrict__ U,
-                                         const double* __restrict__ cu, const double* __restrict__ cv,
-                                         const double* __restrict__ cw, const Geom& g, double dt)
+// Called by the four lanes of a quad like sample_quad; lane 0 stores.
+__device__ __forceinline__ void heun_quad(long p, int k, double* __restrict__ pos, const double* __restrict__ U,
+                                          const double* __restrict__ cu, const double* __restrict__ cv,
+                                          const double* __restrict__ cw, const Geom& g, double dt)
 {
     const double u = U[3 * p + 0], v = U[3 * p + 1], w = U[3 * p + 2];
     double y = pos[3 * p + 0], x = pos[3 * p + 1], z = pos[3 * p + 2];
@@ -468,20 +490,21 @@ __device__ __forceinline__ void heun_one(long p, double* __restrict__ pos, const
     const Loc Ly = locate(y + v * dt, g, false);
     const Loc Lx = locate(x + u * dt, g, false);
     const Loc Lz = locate(z + w * dt, g, false);
+    const double wz = pick4(Lz.w, k);
     const long N = g.N;
+    const long c0 = ((long)Ly.i * N + Lx.i) * N + Lz.i + k;
     const double* cs[3] = {cu, cv, cw};
     double U2[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        double val = 0;
+        double P = 0;
+#pragma unroll
         for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                const double* row = cs[f] + ((Ly.i + a) * N + (Lx.i + b)) * N + Lz.i;
-                const double rv = Lz.w[0] * row[0] + Lz.w[1] * row[1] + Lz.w[2] * row[2] + Lz.w[3] * row[3];
-                val += Ly.w[a] * Lx.w[b] * rv;
-            }
-        U2[f] = val;
+#pragma unroll
+            for (int b = 0; b < 4; b++) P += (Ly.w[a] * Lx.w[b]) * cs[f][c0 + ((long)a * N + b) * N];
+        U2[f] = quad_sum(wz * P);
     }
+    if (k != 0) return;
     y += 0.5 * (v + U2[1]) * dt;
     x += 0.5 * (u + U2[0]) * dt;
     z += 0.5 * (w + U2[2]) * dt;

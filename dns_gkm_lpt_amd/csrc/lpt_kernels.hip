@@ -181,15 +181,16 @@ __global__ __launch_bounds__(256) void sample_kernel(long np, const double* __re
                                                      const double* __restrict__ cv, const double* __restrict__ cw, Geom g,
                                                      double* __restrict__ U, double* __restrict__ G)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < np) sample_one(p, pos, cu, cv, cw, g, U, G);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;  // four threads per tracer
+    if ((t >> 2) < np) sample_quad(t >> 2, (int)(t & 3), pos, cu, cv, cw, g, U, G);
 }
 
 hipError_t launch_sample(long np, const double* pos, const double* const cu[3], const Geom& g, double* U, double* G,
                          hipStream_t s)
 {
     if (np <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, cu[0], cu[1], cu[2], g, U, G);
+    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(4 * np, 256)), dim3(256), 0, s, np, pos, cu[0], cu[1], cu[2], g, U,
+                       G);
     return hipGetLastError();
 }
 
@@ -290,15 +291,16 @@ __global__ __launch_bounds__(256) void heun_kernel(long np, double* __restrict__
                                                    const double* __restrict__ cu, const double* __restrict__ cv,
                                                    const double* __restrict__ cw, Geom g, const double* __restrict__ dtp)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < np) heun_one(p, pos, U, cu, cv, cw, g, *dtp);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;  // four threads per tracer
+    if ((t >> 2) < np) heun_quad(t >> 2, (int)(t & 3), pos, U, cu, cv, cw, g, *dtp);
 }
 
 hipError_t launch_heun(long np, double* pos, const double* U, const double* const cn[3], const Geom& g,
                        const double* dt, hipStream_t s)
 {
     if (np <= 0) return hipSuccess;
-    hipLaunchKernelGGL(heun_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, s, np, pos, U, cn[0], cn[1], cn[2], g, dt);
+    hipLaunchKernelGGL(heun_kernel, dim3(blocks_for(4 * np, 256)), dim3(256), 0, s, np, pos, U, cn[0], cn[1], cn[2], g,
+                       dt);
     return hipGetLastError();
 }
 
@@ -332,8 +334,8 @@ __global__ __launch_bounds__(256) void ens_sample_kernel(const EnsPart* __restri
 {
     const int e = ens_entry(first, n, blockIdx.x);
     const EnsPart P = parts[e];
-    const long p = (long)(blockIdx.x - first[e]) * 256 + threadIdx.x;
-    if (p < P.np) sample_one(p, P.pos, P.c[0], P.c[1], P.c[2], P.g, P.U, P.G);
+    const long t = (long)(blockIdx.x - first[e]) * 256 + threadIdx.x;  // four threads per tracer
+    if ((t >> 2) < P.np) sample_quad(t >> 2, (int)(t & 3), P.pos, P.c[0], P.c[1], P.c[2], P.g, P.U, P.G);
 }
 
 __global__ __launch_bounds__(256) void ens_heun_kernel(const EnsPart* __restrict__ parts, const int* __restrict__ first,
@@ -341,8 +343,8 @@ __global__ __launch_bounds__(256) void ens_heun_kernel(const EnsPart* __restrict
 {
     const int e = ens_entry(first, n, blockIdx.x);
     const EnsPart P = parts[e];
-    const long p = (long)(blockIdx.x - first[e]) * 256 + threadIdx.x;
-    if (p < P.np) heun_one(p, P.pos, P.U, P.c[0], P.c[1], P.c[2], P.g, *P.dt);
+    const long t = (long)(blockIdx.x - first[e]) * 256 + threadIdx.x;  // four threads per tracer
+    if ((t >> 2) < P.np) heun_quad(t >> 2, (int)(t & 3), P.pos, P.U, P.c[0], P.c[1], P.c[2], P.g, *P.dt);
 }
 
 int ens_z_blocks(const Geom& g) { return (int)blocks_for((long)g.nt * g.nt, 4); }
@@ -354,7 +356,7 @@ int ens_xy_blocks(const Geom& g, int axis)
 
 int ens_z_chunks(int nt_max) { return z_chunks(nt_max); }
 
-int ens_particle_blocks(long np) { return (int)blocks_for(np, 256); }
+int ens_particle_blocks(long np) { return (int)blocks_for(4 * np, 256); }  // four threads per tracer
 
 hipError_t launch_ens_prefilter(const ZArgs* zs, const int* zfirst, int nz, int zblk, int zchunks,
                                 const EnsLine* lines, const int* lfirst, int nl, const int lblk[2], hipStream_t s)

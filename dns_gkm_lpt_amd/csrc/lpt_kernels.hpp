@@ -105,8 +105,9 @@ struct EnsPart {  // one member's tracers
     const double* dt;
 };
 // blocks of one entry: z pass (256-thread blocks, a z line per wave), x and y
-// passes (64, axis 1 or 2), particles (256); ens_z_chunks(largest nt) is the
-// z kernel's chunk count (0: nt too large)
+// passes (64, axis 1 or 2), particles (256 threads, four per tracer: 64
+// tracers); ens_z_chunks(largest nt) is the z kernel's chunk count (0: nt too
+// large)
 int ens_z_blocks(const Geom& g);
 int ens_xy_blocks(const Geom& g, int axis);
 int ens_z_chunks(int nt_max);

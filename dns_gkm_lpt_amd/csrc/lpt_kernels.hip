@@ -184,7 +184,8 @@ hipError_t launch_lap_spline(const double* c, const Geom& g, const double* pivot
 // cell blocks, whose stencils share cache lines, meet in the same L2.  At
 // 512^3 with 2^22 tracers the two together bring the bytes fetched past L2 per
 // sample / Heun launch from 12.9 / 15.0 GiB to 4.5 / 4.7 GiB (the u, v, w
-// coefficients are 3.1 GiB); either alone left them at 13-15 GiB.
+// coefficients are 3.1 GiB); the remap with the row-major cell sort left them
+// at 12.8 / 14.9 GiB.
 __device__ __forceinline__ unsigned xcd_block()
 {
     const unsigned b = blockIdx.x, nb = gridDim.x;

@@ -164,8 +164,23 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
     const int s = bid % p.nseg;
     const int pencil = bid / p.nseg;
     const int npb = GEN ? (nc + CB - 1) / CB : nc / CB;  // pencils across b; with GEN the last may reach past nc
-    const int a0 = (pencil / npb) * CA;
-    const int b0 = (pencil % npb) * CB;
+    // z sweep on an exact grid: pencils in 8 x 8 patches, so that the ~64
+    // workgroups an XCD runs at once cover a square of lines whose tangential
+    // edge lines are mostly interior lines of the same patch, read once into
+    // that XCD's L2 (the row order re-read the x edge lines of every strip;
+    // 512^3: z sweep 19.55 -> 19.22 ms mixed, 25.44 -> 24.79 ms FP64).  The
+    // x and y sweeps keep the row order, which measured faster for them.
+    int pa, pb;
+    if (DIR == 2 && !GEN && (npb & 7) == 0 && ((nc / CA) & 7) == 0) {
+        const int t = pencil >> 6, w = pencil & 63, tpb = npb >> 3;
+        pa = (t / tpb) * 8 + (w >> 3);
+        pb = (t % tpb) * 8 + (w & 7);
+    } else {
+        pa = pencil / npb;
+        pb = pencil % npb;
+    }
+    const int a0 = pa * CA;
+    const int b0 = pb * CB;
 
     const int tid = threadIdx.x;
     int tn, ta, tb;

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "gkm_math.hpp"
 #include "lpt_kernels.hpp"
 
 namespace lpt {
@@ -128,17 +129,19 @@ constexpr int kLineBlock = 16;
 // Pass 1: z lines straight from the state (periodic wrap).  Source field q as
 // generateSplines forms it (update.cpp:1202-1216): 0 = rho, 1..3 = velocity
 // W_q / rho, 4 = pressure (rho (gam-1)) (E/rho - |U|^2/2) with U = W/rho.
-// Field q of the state at padded position (a, b, k) of the knot grid.
+// Field q of the state at padded position (a, b, k) of the knot grid; W_q / rho
+// as W_q times the reciprocal of rho (gkm::g_rcp, within 1 ulp of the quotient).
 __device__ __forceinline__ double field_at(const double* __restrict__ W, long plane, int q, double gam, int nc, int a,
                                           int b, int k)
 {
     const long c = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc + wrapi(k - 3, nc);
     const double den = W[c];
     if (q == 0) return den;
-    if (q <= 3) return W[q * plane + c] / den;
-    const double u = W[plane + c] / den, v = W[2 * plane + c] / den, w = W[3 * plane + c] / den;
+    const double ir = gkm::g_rcp(den);
+    if (q <= 3) return W[q * plane + c] * ir;
+    const double u = W[plane + c] * ir, v = W[2 * plane + c] * ir, w = W[3 * plane + c] * ir;
     const double Vsqr = u * u + v * v + w * w;
-    return (den * (gam - 1)) * (W[4 * plane + c] / den - 0.5 * Vsqr);
+    return (den * (gam - 1)) * (W[4 * plane + c] * ir - 0.5 * Vsqr);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,9 +341,9 @@ __device__ __forceinline__ void prefilter_z_line(long line, const ZArgs& z)
         const int p = 64 * j + lane;
         if (p < nt) {
             const long c = ((long)wrapi(a - 3, nc) * nc + wrapi(b - 3, nc)) * nc + wrapi(p - 3, nc);
-            const double den = z.W[c];
+            const double ir = gkm::g_rcp(z.W[c]);
 #pragma unroll
-            for (int q = 0; q < 3; q++) f[q][j] = z.W[(q + 1) * z.plane + c] / den;  // as field_at
+            for (int q = 0; q < 3; q++) f[q][j] = z.W[(q + 1) * z.plane + c] * ir;  // as field_at
         } else {
 #pragma unroll
             for (int q = 0; q < 3; q++) f[q][j] = 0.0;

@@ -1081,10 +1081,23 @@ int ens_build(gkm_ensemble* e, const int t[3])
     for (int d = 0; d < 3; d++) {
         int acc = 0;
         bool exact = true;
+        // table entries longest workgroups first (faces per workgroup along the
+        // sweep, then member order), so that the short ones fill the last round
+        // instead of the long ones trailing it: the persistent step hands the
+        // pencils out in this order (64-member workload: 6.62 -> 6.42 ms per
+        // step); the per-phase launches measured the same either way
+        std::vector<int> order(n);
+        std::vector<int> seg(n);
         for (int m = 0; m < n; m++) {
-            first[(size_t)d * (n + 1) + m] = acc;
-            for (int p = 0; p < 2; p++) args[((size_t)p * 3 + d) * n + m] = ens_sweep_args(e->mem[m], p, d, t[d]);
-            const gkm::SweepArgs& a = args[(size_t)d * n + m];
+            order[m] = m;
+            seg[m] = ens_sweep_args(e->mem[m], 0, d, t[d]).seg;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return seg[x] > seg[y]; });
+        for (int k = 0; k < n; k++) {
+            const int m = order[k];
+            first[(size_t)d * (n + 1) + k] = acc;
+            for (int p = 0; p < 2; p++) args[((size_t)p * 3 + d) * n + k] = ens_sweep_args(e->mem[m], p, d, t[d]);
+            const gkm::SweepArgs& a = args[(size_t)d * n + k];
             acc += gkm::sweep_blocks(d, a.nc, t[d], a.seg);
             exact = exact && gkm::sweep_exact(d, t[d], a.nc, a.seg);
         }

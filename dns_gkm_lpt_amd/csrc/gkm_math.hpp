@@ -244,6 +244,20 @@ __host__ __device__ __forceinline__ T temp_of(const T W[5], T gam, T R)
 // ApplyWENO (update.cpp:556-680): S[s][q] for cells i-2..i+3, out Wl, Wr.
 // Temperatures with (gam-1)/R formed once, and the energy rebuilt with
 // R/(gam-1); both equal the reference's expressions up to rounding.
+// weno5 with the six cells' temperatures given (Tc[s] = temp_gr(S[s], (gam - 1)/R)):
+// two neighbouring faces of a line share five cells and their temperatures
+template <typename T>
+__host__ __device__ __forceinline__ void weno5_t(const T S[6][5], const T Tc[6], const Gas<T>& g, T Wl[5], T Wr[5])
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) weno_pair(S[0][q], S[1][q], S[2][q], S[3][q], S[4][q], S[5][q], g.eps, Wl[q], Wr[q]);
+    T Tl, Tr;
+    weno_pair(Tc[0], Tc[1], Tc[2], Tc[3], Tc[4], Tc[5], g.eps, Tl, Tr);
+    const T Rg = g.R / (g.gam - T(1));
+    Wl[4] = Wl[0] * Rg * Tl + g_div(T(0.5) * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]), Wl[0]);
+    Wr[4] = Wr[0] * Rg * Tr + g_div(T(0.5) * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]), Wr[0]);
+}
+
 template <typename T>
 __host__ __device__ __forceinline__ void weno5(const T S[6][5], const Gas<T>& g, T Wl[5], T Wr[5])
 {
@@ -333,6 +347,29 @@ __host__ __device__ __forceinline__ void half_tab(T m, T i2l, T h0, T h1, T* h)
 // WENO on FP64 stencils, FP32 arithmetic on differences from cell i, the base
 // of both sides (the reconstruction is shift invariant: candidates are affine
 // with unit coefficient sum and the indicators see differences only).
+// weno5_mixed with the six cells' temperatures given (as weno5_t)
+__host__ __device__ __forceinline__ void weno5_mixed_t(const double S[6][5], const double Tc[6], const Gas<double>& g,
+                                                       double Wl[5], double Wr[5])
+{
+    const float eps = (float)g.eps;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double c = S[2][q];
+        float l, r;
+        weno_pair<float>((float)(S[0][q] - c), (float)(S[1][q] - c), 0.f, (float)(S[3][q] - c), (float)(S[4][q] - c),
+                         (float)(S[5][q] - c), eps, l, r);
+        Wl[q] = c + (double)l;
+        Wr[q] = c + (double)r;
+    }
+    float l, r;
+    weno_pair<float>((float)(Tc[0] - Tc[2]), (float)(Tc[1] - Tc[2]), 0.f, (float)(Tc[3] - Tc[2]),
+                     (float)(Tc[4] - Tc[2]), (float)(Tc[5] - Tc[2]), eps, l, r);
+    const double Tl = Tc[2] + (double)l, Tr = Tc[2] + (double)r;
+    const double Rg = g.R / (g.gam - 1.0);
+    Wl[4] = Wl[0] * Rg * Tl + g_div(0.5 * (Wl[1] * Wl[1] + Wl[2] * Wl[2] + Wl[3] * Wl[3]), Wl[0]);
+    Wr[4] = Wr[0] * Rg * Tr + g_div(0.5 * (Wr[1] * Wr[1] + Wr[2] * Wr[2] + Wr[3] * Wr[3]), Wr[0]);
+}
+
 __host__ __device__ __forceinline__ void weno5_mixed(const double S[6][5], const Gas<double>& g, double Wl[5],
                                                      double Wr[5])
 {

@@ -220,7 +220,66 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
     for (int c = -1; c < nchunk; ++c) {
         const int n0 = seg0 + c * CN;  // first face of the chunk (may be < 0 for the lead-in)
 
-        // ---- WENO states of every (face, line) of the chunk into LDS
+        // ---- WENO states of every (face, line) of the chunk into LDS.  In the x
+        // and y sweeps with an even face count a thread takes two neighbouring
+        // faces of a line: their stencils share five of seven cells, loaded and
+        // turned into temperatures once; each face's reconstruction is the same
+        // arithmetic as a face alone (512^3, 4 x 4 x 16 and 2 x 8 x 16 tiles:
+        // x 15.66 -> 14.45 ms, y 17.17 -> 16.42 ms mixed; 22.6 -> 21.2 and
+        // 24.3 -> 23.2 ms FP64).  The z sweep keeps one face per thread: with
+        // pairs, neighbouring lanes read cells two apart along z and it slowed
+        // (19.25 -> 20.16 ms mixed).
+        if constexpr (DIR != 2 && CN % 2 == 0) {
+            constexpr int CH = CN / 2;
+            const double gR = (p.g.gam - 1.0) / p.g.R;
+            for (int i = tid; i < CH * L; i += NT) {
+                const int il = i % L, itn = 2 * (i / L);
+                int ia, ib;
+                LN::coords(il, ia, ib);
+                const int ca = wrapi(a0 + ia, nc), cb = wrapi(b0 + ib, nc);
+                const int fn = n0 + itn;
+                double S[7][5], Tc[7];
+#pragma unroll
+                for (int t = 0; t < 7; t++) {
+                    const long idx = cell_index<DIR>(wrapi(fn - 2 + t, nc), ca, cb, nc);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) S[t][q] = W[q * plane + idx];
+                    Tc[t] = temp_gr<double>(S[t], gR);
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (!MIX && h == 1) {
+                        // FP64: the second face starts from fresh copies of the
+                        // shared cells and temperatures, so the compiler cannot
+                        // merge its arithmetic with the first face's (merged, the
+                        // two faces' second differences and candidates were shared
+                        // and the fused multiply-adds fell differently: a face's
+                        // rounding depended on its place in the pair, and a
+                        // periodic shift of the state no longer commuted with the
+                        // step bit for bit).  The mixed reconstruction works on
+                        // differences from each face's own base cell, so the two
+                        // faces share only the loaded cells and temperatures
+                        // (whose copies made the mixed sweeps spill).
+#pragma unroll
+                        for (int t = 1; t < 7; t++) {
+#pragma unroll
+                            for (int q = 0; q < 5; q++) asm volatile("" : "+v"(S[t][q])::"memory");
+                            asm volatile("" : "+v"(Tc[t])::"memory");
+                        }
+                    }
+                    double Wl[5], Wr[5];
+                    if (MIX)
+                        weno5_mixed_t(S + h, Tc + h, p.g, Wl, Wr);
+                    else
+                        weno5_t<double>(S + h, Tc + h, p.g, Wl, Wr);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) {
+                        sW[sw_at<DIR, CN, L>(q, itn + h, il)] = Wl[q];
+                        sW[sw_at<DIR, CN, L>(5 + q, itn + h, il)] = Wr[q];
+                    }
+                }
+            }
+        } else
         for (int i = tid; i < CN * L; i += NT) {
             int itn, il;
             if (DIR == 2) { itn = i % CN; il = i / CN; }

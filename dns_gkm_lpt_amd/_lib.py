@@ -21,7 +21,7 @@ EXPORTS = [
     "gkm_download_carry", "gkm_upload_state", "gkm_download_state",
     "gkm_upload_state_packed", "gkm_download_state_packed", "gkm_step", "gkm_cfl_dt", "gkm_run", "gkm_sync",
     "gkm_sweep_flux_diff", "gkm_set_sweep_tiles", "gkm_get_sweep_tiles", "gkm_autotune", "gkm_step_noalloc",
-    "gkm_tl_run", "gkm_adjoint_run",
+    "gkm_tl_run", "gkm_adjoint_run", "gkm_adjoint_forward", "gkm_adjoint_backward",
     "gkm_ensemble_create", "gkm_ensemble_destroy", "gkm_ensemble_last_error", "gkm_ensemble_member",
     "gkm_ensemble_set_tiles", "gkm_ensemble_autotune", "gkm_ensemble_run", "gkm_ensemble_dt",
     "gkm_ensemble_set_persistent", "gkm_ensemble_persistent_profile",
@@ -99,6 +99,8 @@ def lib():
         L.gkm_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int * 3), C.POINTER(C.c_double * 9)]
         L.gkm_tl_run.argtypes = [P, C.c_double, C.c_int, _dp, _dp]
         L.gkm_adjoint_run.argtypes = [P, C.c_double, C.c_int, C.c_int, _dp, _dp]
+        L.gkm_adjoint_forward.argtypes = [P, C.c_double, C.c_int, C.c_int]
+        L.gkm_adjoint_backward.argtypes = [P, _dp, _dp]
         L.gkm_ensemble_create.argtypes = [C.c_int, C.POINTER(Config), C.POINTER(P)]
         L.gkm_ensemble_destroy.argtypes = [P]
         L.gkm_ensemble_destroy.restype = None
@@ -251,6 +253,19 @@ class Context:
         lamN = np.ascontiguousarray(lamN, dtype=np.float64)
         out = np.empty((5, self.nt, self.nt, self.nt))
         self._chk(lib().gkm_adjoint_run(self._h, dt, nsteps, every, lamN, out))
+        return out
+
+    def adjoint_forward(self, dt, nsteps, every):
+        """The forward part of adjoint_run: the checkpointed trajectory from the
+        resident state, which ends at W_N (resident)."""
+        self._chk(lib().gkm_adjoint_forward(self._h, dt, nsteps, every))
+
+    def adjoint_backward(self, lamN):
+        """The backward part of adjoint_run from the last adjoint_forward's
+        trajectory (used up): returns lam_0 (padded layout)."""
+        lamN = np.ascontiguousarray(lamN, dtype=np.float64)
+        out = np.empty((5, self.nt, self.nt, self.nt))
+        self._chk(lib().gkm_adjoint_backward(self._h, lamN, out))
         return out
 
     def set_tiles(self, tiles):

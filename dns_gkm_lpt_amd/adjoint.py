@@ -8,9 +8,11 @@ FlowField.cpp:89-105) of the velocity field with uniform density and pressure.
 
     J(u0) = (1/nc^3) sum (1/2) |m_N|^2 / rho_N,   W_N = step^N(W_0(u0))
 
-The gradient is one adjoint run (gkm_adjoint_run: checkpoints every `every`
-steps, each segment recomputed) followed by the transpose of the
-initial-condition map.  KineticEnergy wraps it as a torch.autograd.Function.
+The gradient is one adjoint run (gkm_adjoint_forward: the trajectory with
+checkpoints every `every` steps, ending at W_N, from which J and lam_N are
+formed; gkm_adjoint_backward: each segment but the last recomputed on the way
+back) followed by the transpose of the initial-condition map.  KineticEnergy
+wraps it as a torch.autograd.Function.
 """
 import numpy as np
 
@@ -95,12 +97,10 @@ class Problem:
         W0 = initial_state(u, self.rho, self.p, self.params["gam"])
         with self.context() as c:
             c.upload(W0)
-            for _ in range(self.nsteps):
-                c.step(self.dt)
+            c.adjoint_forward(self.dt, self.nsteps, self.every)
             WN = c.download()
             J = kinetic_energy(WN)
-            c.upload(W0)
-            lam0 = c.adjoint_run(self.dt, self.nsteps, self.every, kinetic_energy_adjoint(WN))
+            lam0 = c.adjoint_backward(kinetic_energy_adjoint(WN))
         return J, initial_state_transpose(lam0, u, self.rho)
 
     def directional(self, u, d):

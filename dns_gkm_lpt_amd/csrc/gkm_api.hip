@@ -58,6 +58,9 @@ struct gkm_ctx {
     double* adj_scratch = nullptr;
     double* adj_vec[2] = {nullptr, nullptr};  // tangent or adjoint vectors, 5 x plane each
     std::vector<double*> adj_states;          // checkpoints and segment states
+    // the trajectory recorded by the last gkm_adjoint_forward (adj_n < 0: none)
+    int adj_n = -1, adj_every = 1;
+    double adj_dt = 0.0;
 };
 
 namespace {
@@ -430,12 +433,45 @@ int gkm_tl_run(gkm_ctx* c, double dt, int nsteps, const double* dW0, double* dWN
     return vec_download(c, c->adj_vec[t], dWN);
 }
 
-int gkm_adjoint_run(gkm_ctx* c, double dt, int nsteps, int every, const double* lamN, double* lam0)
+// The forward trajectory of the adjoint: checkpoints ckpt[k] = W_{k every}, the
+// states W_{s0}..W_{s1-1} of one segment in seg[], and W_N in final_state.
+// The forward pass stores the last segment's states as it goes, so the
+// backward pass recomputes only the segments before it.
+int gkm_adjoint_forward(gkm_ctx* c, double dt, int nsteps, int every)
 {
-    if (!c || !lamN || !lam0 || nsteps < 0 || every < 1) return fail(c, GKM_EINVAL, "bad argument");
+    if (!c || nsteps < 0 || every < 1) return fail(c, GKM_EINVAL, "bad argument");
+    c->adj_n = -1;
     const int nseg = (nsteps + every - 1) / every;
     // checkpoints, the states of one segment, and the final state
     int rc = adj_prepare(c, dt, nseg + every + 1);
+    if (rc) return rc;
+    const size_t sb = sizeof(double) * 5 * (size_t)c->plane;
+    double** ckpt = c->adj_states.data();
+    double** seg = ckpt + nseg;
+    double* final_state = ckpt[nseg + every];
+    const int last0 = (nseg - 1) * every;  // first step of the last segment
+    for (int n = 0; n < nsteps; n++) {
+        if (n % every == 0) HIPCHK(c, hipMemcpyAsync(ckpt[n / every], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+        if (n >= last0) HIPCHK(c, hipMemcpyAsync(seg[n - last0], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+        rc = eulerian_step(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(final_state, c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->adj_n = nsteps;
+    c->adj_every = every;
+    c->adj_dt = dt;
+    return GKM_OK;
+}
+
+int gkm_adjoint_backward(gkm_ctx* c, const double* lamN, double* lam0)
+{
+    if (!c || !lamN || !lam0) return fail(c, GKM_EINVAL, "bad argument");
+    if (c->adj_n < 0) return fail(c, GKM_ENOTREADY, "no forward trajectory (gkm_adjoint_forward)");
+    const int nsteps = c->adj_n, every = c->adj_every;
+    const double dt = c->adj_dt;
+    const int nseg = (nsteps + every - 1) / every;
+    int rc = adj_prepare(c, dt, nseg + every + 1);  // restores the fixed dt of the trajectory
     if (rc) return rc;
     gkm::AdjScratch s{};
     gkm::adj_scratch_carve(c->adj_scratch, c->plane, s);
@@ -443,23 +479,19 @@ int gkm_adjoint_run(gkm_ctx* c, double dt, int nsteps, int every, const double* 
     double** ckpt = c->adj_states.data();
     double** seg = ckpt + nseg;
     double* final_state = ckpt[nseg + every];
-    for (int n = 0; n < nsteps; n++) {
-        if (n % every == 0) HIPCHK(c, hipMemcpyAsync(ckpt[n / every], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
-        rc = eulerian_step(c);
-        if (rc) return rc;
-    }
-    HIPCHK(c, hipMemcpyAsync(final_state, c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
     rc = vec_upload(c, lamN, c->adj_vec[0]);
     if (rc) return rc;
     int l = 0;
     for (int k = nseg - 1; k >= 0; k--) {
         const int s0 = k * every, s1 = (s0 + every < nsteps) ? s0 + every : nsteps;
-        HIPCHK(c, hipMemcpyAsync(c->W[c->cur], ckpt[k], sb, hipMemcpyDeviceToDevice, c->stream));
-        for (int n = s0; n < s1; n++) {
-            HIPCHK(c, hipMemcpyAsync(seg[n - s0], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
-            if (n + 1 < s1) {
-                rc = eulerian_step(c);
-                if (rc) return rc;
+        if (k < nseg - 1) {  // the last segment's states are still there from the forward pass
+            HIPCHK(c, hipMemcpyAsync(c->W[c->cur], ckpt[k], sb, hipMemcpyDeviceToDevice, c->stream));
+            for (int n = s0; n < s1; n++) {
+                HIPCHK(c, hipMemcpyAsync(seg[n - s0], c->W[c->cur], sb, hipMemcpyDeviceToDevice, c->stream));
+                if (n + 1 < s1) {
+                    rc = eulerian_step(c);
+                    if (rc) return rc;
+                }
             }
         }
         for (int n = s1 - 1; n >= s0; n--) {
@@ -470,7 +502,16 @@ int gkm_adjoint_run(gkm_ctx* c, double dt, int nsteps, int every, const double* 
     }
     HIPCHK(c, hipMemcpyAsync(c->W[c->cur], final_state, sb, hipMemcpyDeviceToDevice, c->stream));
     c->lpt.valid = false;
+    c->adj_n = -1;  // the segment buffers now hold the first segment: the trajectory is used up
     return vec_download(c, c->adj_vec[l], lam0);
+}
+
+int gkm_adjoint_run(gkm_ctx* c, double dt, int nsteps, int every, const double* lamN, double* lam0)
+{
+    if (!c || !lamN || !lam0 || nsteps < 0 || every < 1) return fail(c, GKM_EINVAL, "bad argument");
+    const int rc = gkm_adjoint_forward(c, dt, nsteps, every);
+    if (rc) return rc;
+    return gkm_adjoint_backward(c, lamN, lam0);
 }
 
 int gkm_set_sweep_tiles(gkm_ctx* c, const int tile[3])

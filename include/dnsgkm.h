@@ -171,10 +171,19 @@ int gkm_weno_batch(gkm_ctx *ctx, long n, const double *in, int precision, double
  * ignored, output ghost layers are filled periodically.
  * gkm_tl_run: dW_N = (dW_N/dW_0) dW_0.
  * gkm_adjoint_run: lam_0 = (dW_N/dW_0)^T lam_N, with the forward trajectory
- * checkpointed every `every` steps and each segment recomputed on the way back.
+ * checkpointed every `every` steps and each segment but the last recomputed on
+ * the way back.  It is gkm_adjoint_forward followed by gkm_adjoint_backward.
+ * gkm_adjoint_forward: the forward part alone; it leaves W_N resident, so a
+ * caller whose lam_N depends on W_N (a gradient of J(W_N)) downloads W_N and
+ * forms lam_N without a forward run of its own.
+ * gkm_adjoint_backward: the backward part from the trajectory of the last
+ * gkm_adjoint_forward, which it uses up (GKM_ENOTREADY without one); dt,
+ * nsteps and every are the forward call's.
  * Contexts must be GKM_FP64. */
 int gkm_tl_run(gkm_ctx *ctx, double dt, int nsteps, const double *dW0, double *dWN);
 int gkm_adjoint_run(gkm_ctx *ctx, double dt, int nsteps, int every, const double *lamN, double *lam0);
+int gkm_adjoint_forward(gkm_ctx *ctx, double dt, int nsteps, int every);
+int gkm_adjoint_backward(gkm_ctx *ctx, const double *lamN, double *lam0);
 
 /* ---- Ensemble of independent runs (no reference counterpart) -------------
  * n members, each a full context (own grid size, gas, initial condition,

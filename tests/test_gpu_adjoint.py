@@ -70,6 +70,30 @@ def test_dot_product_identity():
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
+def test_forward_backward_split_equals_adjoint_run():
+    """gkm_adjoint_forward + gkm_adjoint_backward (the last segment kept from
+    the forward pass) give adjoint_run's lam_0 bit for bit and leave W_N
+    resident; a second backward without a new forward is refused."""
+    W, P, gas = make_case(NC, seed=23)
+    with gl.Context(NC, P, precision=gl.GKM_FP64) as c:
+        c.upload(W)
+        dt = c.cfl_dt()
+        scale = np.abs(interior(W)).max(axis=(1, 2, 3))
+        lamN = random_state_vector(W.shape, 3, 1.0 / scale)
+        c.upload(W)
+        ref = c.adjoint_run(dt, NSTEPS, EVERY, lamN)
+        WN_run = c.download()
+        c.upload(W)
+        c.adjoint_forward(dt, NSTEPS, EVERY)
+        WN = c.download()
+        lam0 = c.adjoint_backward(lamN)
+        WN_after = c.download()
+        with pytest.raises(gl.GkmError):
+            c.adjoint_backward(lamN)
+    assert np.array_equal(WN, WN_run) and np.array_equal(WN_after, WN_run)
+    assert np.array_equal(interior(lam0), interior(ref))
+
+
 def smooth_state(steps=20):
     """The reference's initial condition (uniform rho and p) advanced a few
     steps, so that density and pressure vary smoothly: at uniform pressure the

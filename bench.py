@@ -57,34 +57,34 @@ ALG_OPS_PER_CELL_UPDATE = 26951
 # Mixed rows: the compensated update (the default); x and y sweeps do not
 # touch the carry.
 MEASURED_TRAFFIC_GIB = {
-    ("mixed", 0, 0, 512): (7.78, 5.00),
-    ("mixed", 0, 2, 512): (7.71, 5.00),
-    ("mixed", 1, 0, 512): (13.17, 5.00),
-    ("mixed", 1, 2, 512): (13.77, 5.00),
-    ("mixed", 2, 0, 512): (28.09, 7.66),
-    ("mixed", 2, 2, 512): (50.00, 7.60),
-    ("fp64", 0, 0, 512): (51.31, 21.23),
-    ("fp64", 0, 2, 512): (68.46, 21.09),
-    ("fp64", 1, 0, 512): (53.96, 17.70),
-    ("fp64", 1, 2, 512): (67.47, 17.58),
-    ("fp64", 2, 0, 512): (63.28, 19.93),
-    ("fp64", 2, 2, 512): (78.54, 19.47),
+    ("mixed", 0, 0, 512): (7.95, 5.00),
+    ("mixed", 0, 2, 512): (22.03, 5.00),
+    ("mixed", 1, 0, 512): (14.68, 5.00),
+    ("mixed", 1, 2, 512): (16.92, 5.00),
+    ("mixed", 2, 0, 512): (26.63, 7.77),
+    ("mixed", 2, 2, 512): (59.64, 7.77),
+    ("fp64", 0, 0, 512): (52.88, 21.27),
+    ("fp64", 0, 2, 512): (65.76, 17.07),
+    ("fp64", 1, 0, 512): (56.53, 16.70),
+    ("fp64", 1, 2, 512): (65.35, 18.61),
+    ("fp64", 2, 0, 512): (59.67, 19.91),
+    ("fp64", 2, 2, 512): (85.34, 19.45),
 }
 # rocprofv3 VALUBusy (% of time the VALU issues) of each sweep launch, keyed
 # like MEASURED_TRAFFIC_GIB; commands and table in profiles/sweep_valu_512.md.
 MEASURED_VALU_BUSY = {
-    ("mixed", 0, 0, 512): 81.9,
-    ("mixed", 0, 2, 512): 76.0,
-    ("mixed", 1, 0, 512): 72.2,
-    ("mixed", 1, 2, 512): 69.8,
-    ("mixed", 2, 0, 512): 69.9,
-    ("mixed", 2, 2, 512): 63.4,
-    ("fp64", 0, 0, 512): 57.3,
-    ("fp64", 0, 2, 512): 54.3,
-    ("fp64", 1, 0, 512): 50.8,
-    ("fp64", 1, 2, 512): 49.1,
-    ("fp64", 2, 0, 512): 53.5,
-    ("fp64", 2, 2, 512): 50.5,
+    ("mixed", 0, 0, 512): 89.4,
+    ("mixed", 0, 2, 512): 75.3,
+    ("mixed", 1, 0, 512): 75.1,
+    ("mixed", 1, 2, 512): 68.1,
+    ("mixed", 2, 0, 512): 71.0,
+    ("mixed", 2, 2, 512): 64.1,
+    ("fp64", 0, 0, 512): 57.6,
+    ("fp64", 0, 2, 512): 52.3,
+    ("fp64", 1, 0, 512): 52.4,
+    ("fp64", 1, 2, 512): 47.7,
+    ("fp64", 2, 0, 512): 54.7,
+    ("fp64", 2, 2, 512): 51.4,
 }
 KERNEL_NAMES = {0: "sweep_kernel<0> (x)", 1: "sweep_kernel<1> (y)", 2: "sweep_kernel<2> (z, with update)",
                 3: "cfl reduction", 4: "spline prefilter (u, v, w)", 5: "particle sample + Heun",

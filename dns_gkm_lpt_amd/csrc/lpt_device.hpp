@@ -458,16 +458,24 @@ __device__ __forceinline__ void sample_quad(long p, int k, const double* __restr
     const double* cs[3] = {cu, cv, cw};
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        double P = 0, Py = 0, Px = 0;  // column k of the value, d/dy and d/dx sums
+        // column k of the value, d/dy and d/dx sums, row by row: the sums over
+        // x of a row with the x weights and their derivatives, then the y
+        // weights (16 live weights where the products wy_a wx_b take 48: 64
+        // VGPRs against 136)
+        double P = 0, Py = 0, Px = 0;
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+        for (int a = 0; a < 4; a++) {
+            double s = 0, sx = 0;
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const double c = cs[f][c0 + ((long)a * N + b) * N];
-                P += (Ly.w[a] * Lx.w[b]) * c;
-                Py += (Ly.d[a] * Lx.w[b]) * c;
-                Px += (Ly.w[a] * Lx.d[b]) * c;
+                s += Lx.w[b] * c;
+                sx += Lx.d[b] * c;
             }
+            P += Ly.w[a] * s;
+            Py += Ly.d[a] * s;
+            Px += Ly.w[a] * sx;
+        }
         const double v = quad_sum(wz * P), gy = quad_sum(wz * Py), gx = quad_sum(wz * Px), gz = quad_sum(dz * P);
         if (k == 0) {
             if (U) U[3 * p + f] = v;

@@ -210,6 +210,16 @@ __device__ __forceinline__ unsigned xcd_block()
     return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Resident tracer waves per SIMD for the gathers at 512^3, 2^22 tracers:
+// more waves in flight fetch more distinct stencils at once and lose cache
+// hits (sample at 64 VGPRs, up to 8 waves: 1.90 ms; capped at 4: 1.81 ms, at
+// 3, 5, 6: 1.86, 1.84, 1.87 ms; Heun at 6 waves: 1.48 ms, capped at 3: 1.40
+// ms, at 2 and 4: 1.47, 1.45 ms).  The cap is dynamic LDS that the kernels do
+// not touch: 160 KB of LDS per CU hold four workgroups of 36 KB or three of
+// 48 KB, a workgroup being one wave per SIMD.
+constexpr unsigned kSampleLdsCap = 36 * 1024;  // 4 waves per SIMD
+constexpr unsigned kHeunLdsCap = 48 * 1024;    // 3 waves per SIMD
+
 __global__ __launch_bounds__(256) void sample_kernel(long np, const double* __restrict__ pos, const double* __restrict__ cu,
                                                      const double* __restrict__ cv, const double* __restrict__ cw, Geom g,
                                                      double* __restrict__ U, double* __restrict__ G)
@@ -222,8 +232,8 @@ hipError_t launch_sample(long np, const double* pos, const double* const cu[3], 
                          hipStream_t s)
 {
     if (np <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(4 * np, 256)), dim3(256), 0, s, np, pos, cu[0], cu[1], cu[2], g, U,
-                       G);
+    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(4 * np, 256)), dim3(256), kSampleLdsCap, s, np, pos, cu[0], cu[1],
+                       cu[2], g, U, G);
     return hipGetLastError();
 }
 
@@ -332,8 +342,8 @@ hipError_t launch_heun(long np, double* pos, const double* U, const double* cons
                        const double* dt, hipStream_t s)
 {
     if (np <= 0) return hipSuccess;
-    hipLaunchKernelGGL(heun_kernel, dim3(blocks_for(4 * np, 256)), dim3(256), 0, s, np, pos, U, cn[0], cn[1], cn[2], g,
-                       dt);
+    hipLaunchKernelGGL(heun_kernel, dim3(blocks_for(4 * np, 256)), dim3(256), kHeunLdsCap, s, np, pos, U, cn[0], cn[1],
+                       cn[2], g, dt);
     return hipGetLastError();
 }
 

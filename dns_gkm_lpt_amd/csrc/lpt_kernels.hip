@@ -101,7 +101,11 @@ struct CoefSet {
 // runs of unrelated workgroups: the passes are bound by how HBM serves runs at
 // a stride of N or N^2 coefficients (0.54 -> 0.53 ms per pass and box at 512^3;
 // more loads in flight per wave, or one box per launch against three, measured
-// the same).
+// the same).  At most two such workgroups run on a CU (dynamic LDS the kernel
+// does not touch, 64 KB of the CU's 160 KB each): fewer rows in flight at once
+// are served faster (x / y pass over u, v, w 1.62 / 1.54 -> 1.59 / 1.50 ms).
+constexpr unsigned kInplaceLdsCap = 64 * 1024;  // 2 waves per SIMD
+
 __global__ __launch_bounds__(256) void prefilter_inplace_kernel(CoefSet cs, unsigned bpf, Geom g, int axis,
                                                                  const double* __restrict__ piv)
 {
@@ -115,10 +119,10 @@ static unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs
 static hipError_t prefilter_xy(const CoefSet& cs, int nf, const Geom& g, const double* pivots, hipStream_t s)
 {
     const unsigned bx = blocks_for((long)g.nt * g.N, 256), by = blocks_for((long)g.N * g.N, 256);
-    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(bx * nf), dim3(256), 0, s, cs, bx, g, 1, pivots);
+    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(bx * nf), dim3(256), kInplaceLdsCap, s, cs, bx, g, 1, pivots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(by * nf), dim3(256), 0, s, cs, by, g, 2, pivots);
+    hipLaunchKernelGGL(prefilter_inplace_kernel, dim3(by * nf), dim3(256), kInplaceLdsCap, s, cs, by, g, 2, pivots);
     return hipGetLastError();
 }
 

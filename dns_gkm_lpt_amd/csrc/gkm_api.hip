@@ -186,7 +186,7 @@ int gkm_create(const gkm_config* cfg, gkm_ctx** out)
         if (hipMalloc(&c->W[b], sbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc state");
     if (hipMalloc(&c->dW, sbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc dW");
     if (cfg->precision == GKM_MIXED) {
-        const size_t cbytes = sizeof(float) * 5 * (size_t)c->plane;
+        const size_t cbytes = sizeof(float) * 5 * (size_t)gkm::carry_plane(c->nc);
         for (int b = 0; b < 2; b++) {
             if (hipMalloc(&c->cmp[b], cbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc carry");
             HIPCHK(c, hipMemsetAsync(c->cmp[b], 0, cbytes, c->stream));
@@ -235,13 +235,13 @@ int gkm_set_compensation(gkm_ctx* c, int on)
     if (!c) return GKM_EINVAL;
     if (c->member) return fail(c, GKM_EINVAL, "an ensemble member keeps its precision's compensation setting");
     if (on && !c->cmp[0]) {
-        const size_t cbytes = sizeof(float) * 5 * (size_t)c->plane;
+        const size_t cbytes = sizeof(float) * 5 * (size_t)gkm::carry_plane(c->nc);
         for (int b = 0; b < 2; b++)
             if (hipMalloc(&c->cmp[b], cbytes) != hipSuccess) return fail(c, GKM_ENOMEM, "hipMalloc carry");
     }
     if (on && !c->compensate) {
         // the resident state starts with no carried error
-        HIPCHK(c, hipMemsetAsync(c->cmp[c->cur], 0, sizeof(float) * 5 * (size_t)c->plane, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->cmp[c->cur], 0, sizeof(float) * 5 * (size_t)gkm::carry_plane(c->nc), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->compensate = on != 0;
@@ -255,9 +255,18 @@ int gkm_download_carry(gkm_ctx* c, float* out)
     if (!c || !out) return fail(c, GKM_EINVAL, "null argument");
     if (!c->have_state) return fail(c, GKM_ENOTREADY, "no state uploaded");
     if (!c->compensate) return fail(c, GKM_EINVAL, "compensation is off");
-    HIPCHK(c, hipMemcpyAsync(out, c->cmp[c->cur], sizeof(float) * 5 * (size_t)c->plane, hipMemcpyDeviceToHost,
-                             c->stream));
+    // the device keeps the z sweep's blocked layout (gkm::carry_index); out is
+    // in the state's (y, x, z) order
+    const int nc = c->nc;
+    const long cp = gkm::carry_plane(nc);
+    std::vector<float> blk((size_t)5 * cp);
+    HIPCHK(c, hipMemcpyAsync(blk.data(), c->cmp[c->cur], sizeof(float) * blk.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 5; q++)
+        for (int y = 0; y < nc; y++)
+            for (int x = 0; x < nc; x++)
+                for (int z = 0; z < nc; z++)
+                    out[q * c->plane + ((long)y * nc + x) * nc + z] = blk[q * cp + gkm::carry_index(y, x, z, nc)];
     return GKM_OK;
 }
 
@@ -272,7 +281,7 @@ int gkm_upload_state(gkm_ctx* c, const double* const W[5])
     int rc = timed(c, 7, [&] { return gkm::launch_unpack(c->stage, c->W[c->cur], c->nc, c->stream); });
     if (rc) return rc;
     if (c->cmp[c->cur])  // a new state carries no rounding error
-        HIPCHK(c, hipMemsetAsync(c->cmp[c->cur], 0, sizeof(float) * 5 * (size_t)c->plane, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->cmp[c->cur], 0, sizeof(float) * 5 * (size_t)gkm::carry_plane(c->nc), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     c->lpt.valid = false;  // a spline built earlier described the previous state

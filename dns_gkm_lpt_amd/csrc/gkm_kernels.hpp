@@ -16,6 +16,7 @@ struct SweepArgs {
     // FP64 update: cmp_in holds, per cell and variable, the rounding error the
     // previous step's update left out of W (exact sum = W + cmp_in); the update
     // adds it to the increment and stores its own error in cmp_out (TwoSum).
+    // Layout: carry_plane(nc) floats per variable, cell at carry_index.
     const float* cmp_in;
     float* cmp_out;
     const double* dt;    // device scalar (graph-capturable)
@@ -28,6 +29,19 @@ struct SweepArgs {
     double dx, inv_dx;
     Gas<double> g;
 };
+
+// Layout of the compensated update's FP32 carry, which only the z sweep's
+// update reads and writes: blocks of 16 cells along z, within a block the
+// x-y plane x-major, cell (y, x, z) at ((z / 16 nc + x) nc + y) 16 + z mod 16.
+// A wave of the 16 x 4 x 4 z tile updates 16 z cells of 4 lines of
+// consecutive y, so each of its carry loads and stores is one 256 B run (the
+// state's layout gives four 64 B runs, halves of lines that the chunk along z
+// after it fills).  Blocks past nc (nc not a multiple of 16) leave gaps.
+__host__ __device__ __forceinline__ long carry_plane(int nc) { return (long)nc * nc * ((nc + 15) & ~15); }
+__host__ __device__ __forceinline__ long carry_index(int y, int x, int z, int nc)
+{
+    return (((long)(z >> 4) * nc + x) * nc + y) * 16 + (z & 15);
+}
 
 // Sweep tile shapes: CN faces along the sweep axis x CA x CB tangential lines
 // (256 threads, one face each).  A chunk reconstructs CN x (CA CB + 2 CA + 2 CB)

@@ -344,14 +344,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& p, int bid, int nblk
                 if (DIR == 0 || p.mode == 0) p.dW[o] = dF;
                 else if (DIR == 1) p.dW[o] += dF;
                 else if (CMP == 1 || (CMP < 0 && p.cmp_out)) {
-                    // W^{n+1} + e = W^n + (increment + carried error), exactly (TwoSum)
+                    // W^{n+1} + e = W^n + (increment + carried error), exactly (TwoSum);
+                    // z sweep: n = z, a = x, b = y
+                    const long co = q * carry_plane(nc) + carry_index(b0 + tb, a0 + ta, n0 + tn, nc);
                     const double w = W[o];
-                    const double y = (double)p.cmp_in[o] - p.inv_dx * (p.dW[o] + dF);
+                    const double y = (double)p.cmp_in[co] - p.inv_dx * (p.dW[o] + dF);
                     const double t = w + y;
                     const double yv = t - w;
                     const double e = (w - (t - yv)) + (y - yv);
                     p.Wout[o] = t;
-                    p.cmp_out[o] = (float)e;
+                    p.cmp_out[co] = (float)e;
                 } else {
                     p.Wout[o] = W[o] - p.inv_dx * (p.dW[o] + dF);
                 }

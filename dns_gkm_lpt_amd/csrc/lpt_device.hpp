@@ -240,11 +240,15 @@ __device__ __forceinline__ double lane_of(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// natural-spline solve of one z line: f holds the M knot values (zero beyond M), out
-// receives the M + 2 coefficients.  Called by every lane of a full wave.
-template <int K>
-__device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double* __restrict__ piv,
-                                            double* __restrict__ out)
+// natural-spline solve of NF z lines of one knot grid: f[q] holds the M knot
+// values of line q (zero beyond M), out[q] receives its M + 2 coefficients.
+// The NF lines go through each step together: every line's operations are
+// those of a solve on its own, and the lines' independent scan chains give
+// the VALU instructions to issue while one chain waits.  Called by every lane
+// of a full wave.
+template <int K, int NF>
+__device__ __forceinline__ void zline_solve_n(double (&f)[NF][K], int M, const double* __restrict__ piv,
+                                              double* const (&out)[NF])
 {
     const int lane = threadIdx.x & 63, n = M - 2;
     // chunk j (rows 64 j - 1 .. 64 j + 62) is "inner" when j >= 1 (rows >= 63,
@@ -255,68 +259,126 @@ __device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double*
     constexpr bool kInner = K >= 3 && K < kMaxZChunks;
     const double pc = kInner ? piv[n - 1 < 63 ? n - 1 : 63] : 0.0;
     const ScanW sw = scan_weights(-pc, lane);
-    const double c1 = lane_of(f[0], 0);
-    double cM = 0.0;
+    double c1[NF], cM[NF], carry[NF];
+#pragma unroll
+    for (int q = 0; q < NF; q++) {
+        c1[q] = lane_of(f[q][0], 0);
+        cM[q] = 0.0;
+        carry[q] = 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < K; j++)
-        if (j == (M - 1) / 64) cM = lane_of(f[j], (M - 1) % 64);
-    double carry = 0.0;
+        if (j == (M - 1) / 64)
+#pragma unroll
+            for (int q = 0; q < NF; q++) cM[q] = lane_of(f[q][j], (M - 1) % 64);
 #pragma unroll
     for (int j = 0; j < K; j++) {  // forward: r = p - 1 for position p = 64 j + lane
         if (kInner && j >= 1 && 64 * j + 62 <= n - 2) {
-            f[j] = sw.wc * carry + wave_scan_const(pc * (6.0 * f[j]), sw);
-            carry = lane_of(f[j], 63);
+#pragma unroll
+            for (int q = 0; q < NF; q++) f[q][j] = sw.wc * carry[q] + wave_scan_const(pc * (6.0 * f[q][j]), sw);
+#pragma unroll
+            for (int q = 0; q < NF; q++) carry[q] = lane_of(f[q][j], 63);
             continue;
         }
         const int r = 64 * j + lane - 1;
-        double A = 1.0, B = 0.0;
+        double A[NF], B[NF];
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            A[q] = 1.0;
+            B[q] = 0.0;
+        }
         if (r >= 0 && r < n) {
             const double pv = piv[r];
-            double rhs = 6.0 * f[j];
-            if (r == 0) rhs -= c1;
-            if (r == n - 1) rhs -= cM;
-            A = -pv;
-            B = pv * rhs;
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                double rhs = 6.0 * f[q][j];
+                if (r == 0) rhs -= c1[q];
+                if (r == n - 1) rhs -= cM[q];
+                A[q] = -pv;
+                B[q] = pv * rhs;
+            }
         }
-        wave_scan(A, B);
-        f[j] = A * carry + B;
-        carry = lane_of(f[j], 63);
+#pragma unroll
+        for (int q = 0; q < NF; q++) wave_scan(A[q], B[q]);
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            f[q][j] = A[q] * carry[q] + B[q];
+            carry[q] = lane_of(f[q][j], 63);
+        }
     }
-    double xl = 0.0;  // x_{n-1} = dp_{n-1}
+    double xl[NF], x0[NF];  // x_{n-1} = dp_{n-1}; x_0
+#pragma unroll
+    for (int q = 0; q < NF; q++) {
+        xl[q] = 0.0;
+        x0[q] = 0.0;
+        carry[q] = 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < K; j++)
-        if (j == n / 64) xl = lane_of(f[j], n % 64);
-    carry = 0.0;
-    double x0 = 0.0;
+        if (j == n / 64)
+#pragma unroll
+            for (int q = 0; q < NF; q++) xl[q] = lane_of(f[q][j], n % 64);
 #pragma unroll
     for (int j = K - 1; j >= 0; j--) {  // backward on reversed chunks: lane l holds r = 64 j + 62 - l
-        const double d = __shfl(f[j], 63 - lane);
+        double d[NF];
+#pragma unroll
+        for (int q = 0; q < NF; q++) d[q] = __shfl(f[q][j], 63 - lane);
         const int r = 64 * j + 62 - lane;
         if (kInner && j >= 1 && 64 * j + 62 <= n - 2) {
-            const double x = sw.wc * carry + wave_scan_const(d, sw);
-            carry = lane_of(x, 63);
-            out[r + 2] = x;
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                const double x = sw.wc * carry[q] + wave_scan_const(d[q], sw);
+                carry[q] = lane_of(x, 63);
+                out[q][r + 2] = x;
+            }
             continue;
         }
-        double A = 1.0, B = 0.0;
-        if (r >= 0 && r < n) {
-            A = r == n - 1 ? 0.0 : -piv[r];
-            B = d;
+        double A[NF], B[NF];
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            A[q] = 1.0;
+            B[q] = 0.0;
         }
-        wave_scan(A, B);
-        const double x = A * carry + B;
-        carry = lane_of(x, 63);
-        if (r >= 0 && r < n) out[r + 2] = x;
-        if (j == 0) x0 = lane_of(x, 62);  // x_0 = c_2
+        if (r >= 0 && r < n) {
+            const double a = r == n - 1 ? 0.0 : -piv[r];
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                A[q] = a;
+                B[q] = d[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NF; q++) wave_scan(A[q], B[q]);
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            const double x = A[q] * carry[q] + B[q];
+            carry[q] = lane_of(x, 63);
+            if (r >= 0 && r < n) out[q][r + 2] = x;
+            if (j == 0) x0[q] = lane_of(x, 62);  // x_0 = c_2
+        }
     }
     if (lane == 0) {
-        out[1] = c1;
-        out[M] = cM;
-        out[0] = 2.0 * c1 - x0;
-        out[M + 1] = 2.0 * cM - xl;
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            out[q][1] = c1[q];
+            out[q][M] = cM[q];
+            out[q][0] = 2.0 * c1[q] - x0[q];
+            out[q][M + 1] = 2.0 * cM[q] - xl[q];
+        }
     }
 }
 
+// one z line (see zline_solve_n)
+template <int K>
+__device__ __forceinline__ void zline_solve(double (&f)[K], int M, const double* __restrict__ piv,
+                                            double* __restrict__ out)
+{
+    double g[1][K];
+#pragma unroll
+    for (int j = 0; j < K; j++) g[0][j] = f[j];
+    double* const o[1] = {out};
+    zline_solve_n<K, 1>(g, M, piv, o);
+}
 
 template <int K>
 __device__ __forceinline__ void prefilter_z_line(long line, const ZArgs& z)
@@ -349,8 +411,8 @@ __device__ __forceinline__ void prefilter_z_line(long line, const ZArgs& z)
             for (int q = 0; q < 3; q++) f[q][j] = 0.0;
         }
     }
-#pragma unroll
-    for (int q = 0; q < 3; q++) zline_solve<K>(f[q], nt, z.piv, z.coef[q] + rowoff);
+    double* const out[3] = {z.coef[0] + rowoff, z.coef[1] + rowoff, z.coef[2] + rowoff};
+    zline_solve_n<K, 3>(f, nt, z.piv, out);
 }
 
 // in place along z (stride 1; values at the knots of the coefficient box), x

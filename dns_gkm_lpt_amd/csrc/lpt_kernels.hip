@@ -101,10 +101,11 @@ struct CoefSet {
 // runs of unrelated workgroups: the passes are bound by how HBM serves runs at
 // a stride of N or N^2 coefficients (0.54 -> 0.53 ms per pass and box at 512^3;
 // more loads in flight per wave, or one box per launch against three, measured
-// the same).  At most two such workgroups run on a CU (dynamic LDS the kernel
-// does not touch, 64 KB of the CU's 160 KB each): fewer rows in flight at once
-// are served faster (x / y pass over u, v, w 1.62 / 1.54 -> 1.59 / 1.50 ms).
-constexpr unsigned kInplaceLdsCap = 64 * 1024;  // 2 waves per SIMD
+// the same).  One such workgroup runs on a CU at a time (dynamic LDS the
+// kernel does not touch, 96 KB of the CU's 160 KB): fewer rows in flight at
+// once are served faster (x / y pass over u, v, w at three waves per SIMD
+// 1.62 / 1.54 ms, at two 1.57 / 1.50, at one 1.44 / 1.41).
+constexpr unsigned kInplaceLdsCap = 96 * 1024;  // 1 wave per SIMD
 
 __global__ __launch_bounds__(256) void prefilter_inplace_kernel(CoefSet cs, unsigned bpf, Geom g, int axis,
                                                                  const double* __restrict__ piv)

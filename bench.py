@@ -57,12 +57,12 @@ ALG_OPS_PER_CELL_UPDATE = 26951
 # Mixed rows: the compensated update (the default); x and y sweeps do not
 # touch the carry.
 MEASURED_TRAFFIC_GIB = {
-    ("mixed", 0, 0, 512): (7.95, 5.00),
-    ("mixed", 0, 2, 512): (22.03, 5.00),
-    ("mixed", 1, 0, 512): (14.68, 5.00),
-    ("mixed", 1, 2, 512): (16.92, 5.00),
-    ("mixed", 2, 0, 512): (26.63, 7.77),
-    ("mixed", 2, 2, 512): (59.64, 7.77),
+    ("mixed", 0, 0, 512): (7.96, 5.00),
+    ("mixed", 0, 2, 512): (21.88, 5.00),
+    ("mixed", 1, 0, 512): (14.59, 5.00),
+    ("mixed", 1, 2, 512): (17.22, 5.00),
+    ("mixed", 2, 0, 512): (23.18, 7.50),
+    ("mixed", 2, 2, 512): (53.09, 7.85),
     ("fp64", 0, 0, 512): (52.88, 21.27),
     ("fp64", 0, 2, 512): (65.76, 17.07),
     ("fp64", 1, 0, 512): (56.53, 16.70),

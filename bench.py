@@ -73,12 +73,12 @@ MEASURED_TRAFFIC_GIB = {
 # rocprofv3 VALUBusy (% of time the VALU issues) of each sweep launch, keyed
 # like MEASURED_TRAFFIC_GIB; commands and table in profiles/sweep_valu_512.md.
 MEASURED_VALU_BUSY = {
-    ("mixed", 0, 0, 512): 89.4,
+    ("mixed", 0, 0, 512): 89.0,
     ("mixed", 0, 2, 512): 75.3,
-    ("mixed", 1, 0, 512): 75.1,
+    ("mixed", 1, 0, 512): 75.2,
     ("mixed", 1, 2, 512): 68.1,
-    ("mixed", 2, 0, 512): 71.0,
-    ("mixed", 2, 2, 512): 64.1,
+    ("mixed", 2, 0, 512): 71.7,
+    ("mixed", 2, 2, 512): 65.7,
     ("fp64", 0, 0, 512): 57.6,
     ("fp64", 0, 2, 512): 52.3,
     ("fp64", 1, 0, 512): 52.4,
